@@ -1,0 +1,687 @@
+// pbf_capi.hip -- the C ABI of include/pbf.h: context lifetime, validation,
+// device allocation, step orchestration and read-back.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "pbf_internal.h"
+
+using pbf::Ctx;
+using pbf::SceneGrid;
+using pbf::SimConst;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(Ctx* x, int code, const std::string& msg) {
+  if (x) x->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((x), PBF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool is_finite_f(float v) { return isfinite(v) != 0; }
+
+int validate_params(const pbf_params* p, std::string* why) {
+  char buf[256];
+  if (!p) {
+    *why = "params is NULL";
+    return PBF_ERR_INVALID;
+  }
+  const float vals[] = {p->h,        p->rest_density, p->epsilon,   p->k_tensile, p->dq,
+                        p->xsph_c,   p->vort_eps,     p->gravity,   p->dt,        p->cell_scale,
+                        p->poly6_coef, p->spiky_coef, p->eps_d};
+  for (float v : vals)
+    if (!is_finite_f(v)) {
+      *why = "non-finite parameter";
+      return PBF_ERR_INVALID;
+    }
+  if (!(p->h > 0.0f) || !(p->rest_density > 0.0f) || !(p->dt > 0.0f) || !(p->cell_scale >= 1.0f) ||
+      !(p->epsilon >= 0.0f) || !(p->eps_d >= 0.0f)) {
+    *why = "h, rest_density, dt must be > 0, cell_scale >= 1, epsilon and eps_d >= 0";
+    return PBF_ERR_INVALID;
+  }
+  if (p->iterations < 1 || p->iterations > 1000 || p->n_tensile < 0 || p->n_tensile > 16 ||
+      p->max_neighbors < 1 || p->max_neighbors > 4096) {
+    snprintf(buf, sizeof buf,
+             "iterations %d (1..1000), n_tensile %d (0..16), max_neighbors %d (1..4096)",
+             p->iterations, p->n_tensile, p->max_neighbors);
+    *why = buf;
+    return PBF_ERR_INVALID;
+  }
+  return PBF_OK;
+}
+
+int validate_box(const float* lo, const float* hi, std::string* why) {
+  for (int a = 0; a < 3; a++)
+    if (!is_finite_f(lo[a]) || !is_finite_f(hi[a]) || !(hi[a] > lo[a])) {
+      *why = "box must satisfy lo < hi on every axis";
+      return PBF_ERR_INVALID;
+    }
+  return PBF_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+void free_device(Ctx* x) {
+  pbf::DevState& d = x->d;
+  dfree(d.posA); dfree(d.posB); dfree(d.velA); dfree(d.velB);
+  dfree(d.predP); dfree(d.predQ); dfree(d.vort); dfree(d.vxsph);
+  dfree(d.cell); dfree(d.arrival); dfree(d.cellSorted); dfree(d.tmpId);
+  dfree(d.cellCount); dfree(d.cellStart); dfree(d.blockSums);
+  dfree(d.nbr); dfree(d.ncnt); dfree(d.flags);
+  dfree(d.grids); dfree(d.slotScene); dfree(d.stage);
+  x->alloc_n = 0;
+  x->alloc_cells = 0;
+}
+
+template <class T>
+hipError_t dalloc(T*& p, size_t count) {
+  return hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+int ensure_device(Ctx* x) {
+  if (x->alloc_n == (size_t)x->n && x->alloc_cells == (size_t)x->ncell && x->d.posA) return PBF_OK;
+  free_device(x);
+  pbf::DevState& d = x->d;
+  const size_t n = (size_t)x->n, np = (size_t)x->npad, C = (size_t)x->ncell;
+  const size_t K = (size_t)x->p.max_neighbors;
+  const size_t nb = (C + 1023) / 1024 + 1;
+  hipError_t e = hipSuccess;
+#define DA(ptr, cnt) \
+  if (e == hipSuccess) e = dalloc(ptr, cnt)
+  DA(d.posA, np); DA(d.posB, np); DA(d.velA, np); DA(d.velB, np);
+  DA(d.predP, np); DA(d.predQ, np); DA(d.vort, np); DA(d.vxsph, np);
+  DA(d.cell, np); DA(d.arrival, np); DA(d.cellSorted, np); DA(d.tmpId, np);
+  DA(d.cellCount, C + 4); DA(d.cellStart, C + 4); DA(d.blockSums, nb);
+  DA(d.nbr, np * K); DA(d.ncnt, np); DA(d.flags, 64);
+  DA(d.grids, (size_t)x->nscenes); DA(d.slotScene, np); DA(d.stage, 4 * np);
+#undef DA
+  if (e != hipSuccess) {
+    free_device(x);
+    return fail(x, PBF_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+  }
+  x->alloc_n = n;
+  x->alloc_cells = C;
+  return PBF_OK;
+}
+
+int finish_create(Ctx* x, pbf_ctx** out) {
+  uint64_t total = 0;
+  for (auto& g : x->grids) {
+    if (total > (1ull << 30)) break;
+    g.cell_base = (uint32_t)total;
+    total += (uint64_t)g.dims[0] * (uint64_t)g.dims[1] * (uint64_t)g.dims[2];
+  }
+  if (total > (1ull << 30)) {
+    std::string m = "grid too large (more than 2^30 cells): enlarge cell_scale or shrink the box";
+    delete x;
+    return fail(nullptr, PBF_ERR_INVALID, m);
+  }
+  x->ncell = (uint32_t)total;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    std::string m = std::string("HIP unavailable: ") + hipGetErrorString(e);
+    delete x;
+    return fail(nullptr, PBF_ERR_HIP, m);
+  }
+  x->device = dev;
+  *out = reinterpret_cast<pbf_ctx*>(x);
+  return PBF_OK;
+}
+
+Ctx* C(pbf_ctx* c) { return reinterpret_cast<Ctx*>(c); }
+const Ctx* C(const pbf_ctx* c) { return reinterpret_cast<const Ctx*>(c); }
+
+// Per-pass HIP events of a profiled step (recorded on the context stream).
+struct PassEvents {
+  std::vector<hipEvent_t> ev;  // 2 per launch
+  std::vector<int> pass;       // PBF_PASS_* of each launch
+  int used = 0;
+};
+
+int mark(Ctx* x, PassEvents* pe, int pass, bool begin) {
+  if (!pe) return PBF_OK;
+  if (begin) {
+    if ((size_t)(pe->used + 1) * 2 > pe->ev.size()) {
+      hipEvent_t a, b;
+      HIPCHK(x, hipEventCreate(&a));
+      HIPCHK(x, hipEventCreate(&b));
+      pe->ev.push_back(a);
+      pe->ev.push_back(b);
+      pe->pass.push_back(pass);
+    }
+    pe->pass[(size_t)pe->used] = pass;
+    HIPCHK(x, hipEventRecord(pe->ev[(size_t)pe->used * 2], x->stream));
+  } else {
+    HIPCHK(x, hipEventRecord(pe->ev[(size_t)pe->used * 2 + 1], x->stream));
+    pe->used++;
+  }
+  return PBF_OK;
+}
+
+#define PASS(pass, call)                               \
+  do {                                                 \
+    int r_ = mark(x, pe, (pass), true);                \
+    if (r_ != PBF_OK) return r_;                       \
+    HIPCHK(x, call);                                   \
+    if ((r_ = mark(x, pe, (pass), false)) != PBF_OK) return r_; \
+  } while (0)
+
+// One step, optionally stopping after a debug stage (0 = full step).
+int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr) {
+  pbf::DevState& d = x->d;
+  PASS(PBF_PASS_PREDICT, pbf::launch_predict(*x, c));
+  PASS(PBF_PASS_SCAN, pbf::launch_scan(*x));
+  PASS(PBF_PASS_SORT, pbf::launch_scatter_rank(*x));
+  x->dbg_pred = d.predQ;
+  x->dbg_vel = d.velB;
+  if (stage == PBF_STAGE_PREDICT) return PBF_OK;
+  PASS(PBF_PASS_NEIGHBORS, pbf::launch_nbr_build(*x, c));
+  x->has_nbrs = true;
+  if (stage == PBF_STAGE_NEIGHBORS) return PBF_OK;
+  float4* cur = d.predQ;
+  float4* other = d.predP;
+  for (int it = 0; it < x->p.iterations; it++) {
+    PASS(PBF_PASS_LAMBDA, pbf::launch_lambda(*x, c, cur, other));
+    if (stage == PBF_STAGE_LAMBDA0 && it == 0) {
+      x->dbg_pred = other;
+      return PBF_OK;
+    }
+    PASS(PBF_PASS_DELTAP, pbf::launch_deltap(*x, c, other, cur, it == x->p.iterations - 1));
+    if (stage == PBF_STAGE_DELTAP0 && it == 0) {
+      x->dbg_pred = cur;
+      return PBF_OK;
+    }
+  }
+  x->dbg_pred = cur;
+  if (stage == PBF_STAGE_SOLVE) return PBF_OK;
+  PASS(PBF_PASS_XSPH, pbf::launch_xsph(*x, c, cur, other));
+  x->dbg_pred = other;
+  if (stage == PBF_STAGE_XSPH) return PBF_OK;
+  PASS(PBF_PASS_VORTICITY, pbf::launch_vort_commit(*x, c, other));
+  x->dbg_vel = d.velA;
+  return PBF_OK;
+}
+
+int check_ready(Ctx* x) {
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (!x->loaded) return fail(x, PBF_ERR_STATE, "no particles loaded");
+  HIPCHK(x, hipSetDevice(x->device));
+  return PBF_OK;
+}
+
+int copy_out(Ctx* x, const float* dsrc, void* host, size_t bytes) {
+  HIPCHK(x, hipMemcpyAsync(host, dsrc, bytes, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  return PBF_OK;
+}
+
+}  // namespace
+
+namespace pbf {
+
+// Derived constants in float, each in the reference's evaluation order:
+// intpow by repeated multiplication (src/bsdf.h:49-57), the tensile scale
+// 1/poly6((0,0,dq*h)) (src/particles.cpp:151), 1/rho0 for the vector
+// divisions (vector3D.h:100-102), g*dt (:180), dt*eps_v (:242).
+void derive_constants(const pbf_params& p, float dt, SimConst* c) {
+  c->h = p.h;
+  c->h2 = p.h * p.h;
+  float h6 = p.h, h9 = p.h;
+  for (int k = 1; k < 6; k++) h6 = h6 * p.h;
+  for (int k = 1; k < 9; k++) h9 = h9 * p.h;
+  c->h6 = h6;
+  c->h9 = h9;
+  c->c1 = p.poly6_coef;
+  c->m3c2 = -3.0f * p.spiky_coef;
+  c->eps_d = p.eps_d;
+  const float z = p.dq * p.h, zero = 0.0f;
+  const float r2 = (zero * zero + zero * zero) + z * z;
+  float w = 0.0f;
+  if (r2 < c->h2) {
+    const float t = c->h2 - r2;
+    w = (c->c1 * ((t * t) * t)) / c->h9;
+  }
+  c->S = 1.0f / w;
+  c->rho0 = p.rest_density;
+  c->inv_rho0 = 1.0f / p.rest_density;
+  c->eps = p.epsilon;
+  c->negk = -p.k_tensile;
+  c->xc = p.xsph_c;
+  c->dt = dt;
+  c->dtv = dt * p.vort_eps;
+  c->inv_dt = 1.0f / dt;
+  c->gdt = p.gravity * dt;
+  c->n_tensile = p.n_tensile;
+}
+
+void derive_grid(const pbf_params& p, const float* lo, const float* hi, SceneGrid* g) {
+  const float cell = p.h * p.cell_scale;
+  const float inv = 1.0f / cell;
+  for (int a = 0; a < 3; a++) {
+    g->lo[a] = lo[a];
+    g->hi[a] = hi[a];
+    g->lo_e[a] = lo[a] + p.eps_d;
+    g->hi_e[a] = hi[a] - p.eps_d;
+    const float ext = floorf((hi[a] - lo[a]) * inv);
+    g->dims[a] = (ext >= 0.0f && ext < 1e6f) ? (int)ext + 1 : 1;
+  }
+  g->inv_cell = inv;
+  g->stop = p.wall_stop_mask;
+  g->cell_base = 0;
+}
+
+}  // namespace pbf
+
+extern "C" {
+
+PBF_API void pbf_params_default(pbf_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->h = 0.3f;
+  p->rest_density = 1000.0f;
+  p->iterations = 12;
+  p->epsilon = 2.0f;
+  p->k_tensile = 0.0001f;
+  p->n_tensile = 4;
+  p->dq = 0.1f;
+  p->xsph_c = 0.001f;
+  p->vort_eps = 0.001f;
+  p->gravity = 10.0f;
+  p->dt = 0.016f;
+  p->box_lo[0] = -1.0f; p->box_lo[1] = 0.0f;  p->box_lo[2] = -1.0f;
+  p->box_hi[0] = 1.0f;  p->box_hi[1] = 1.49f; p->box_hi[2] = 1.0f;
+  p->wall_stop_mask = (1u << 3) | (1u << 5);
+  p->cell_scale = 1.0f + 1.0f / 1024.0f;
+  p->max_neighbors = 96;
+  p->poly6_coef = 1.56668147106f;
+  p->spiky_coef = 4.774648292756860f;
+  p->eps_d = 0.00000000001f;
+}
+
+PBF_API int pbf_create(const pbf_params* p, pbf_ctx** out) {
+  std::string why;
+  if (!out) return fail(nullptr, PBF_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int rc = validate_params(p, &why);
+  if (rc == PBF_OK) rc = validate_box(p->box_lo, p->box_hi, &why);
+  if (rc != PBF_OK) return fail(nullptr, rc, why);
+  Ctx* x = new Ctx();
+  x->p = *p;
+  x->nscenes = 1;
+  x->counts.assign(1, 0);
+  x->offsets.assign(1, 0);
+  x->grids.resize(1);
+  pbf::derive_grid(*p, p->box_lo, p->box_hi, &x->grids[0]);
+  return finish_create(x, out);
+}
+
+PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t* counts,
+                             const float* box_lo, const float* box_hi, pbf_ctx** out) {
+  std::string why;
+  if (!out) return fail(nullptr, PBF_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int rc = validate_params(p, &why);
+  if (rc != PBF_OK) return fail(nullptr, rc, why);
+  if (nscenes < 1 || !counts || !box_lo || !box_hi)
+    return fail(nullptr, PBF_ERR_INVALID, "nscenes >= 1 and counts/box_lo/box_hi are required");
+  Ctx* x = new Ctx();
+  x->p = *p;
+  x->nscenes = nscenes;
+  x->counts.assign(counts, counts + nscenes);
+  x->offsets.resize(nscenes);
+  x->grids.resize(nscenes);
+  int64_t tot = 0;
+  for (int s = 0; s < nscenes; s++) {
+    if (counts[s] < 1 || validate_box(box_lo + 3 * s, box_hi + 3 * s, &why) != PBF_OK) {
+      delete x;
+      return fail(nullptr, PBF_ERR_INVALID,
+                  "scene " + std::to_string(s) + ": count must be >= 1 and lo < hi");
+    }
+    x->offsets[s] = tot;
+    tot += counts[s];
+    pbf::derive_grid(*p, box_lo + 3 * s, box_hi + 3 * s, &x->grids[s]);
+  }
+  if (tot >= (int64_t)1 << 31) {
+    delete x;
+    return fail(nullptr, PBF_ERR_INVALID, "more than 2^31 particles");
+  }
+  return finish_create(x, out);
+}
+
+PBF_API int pbf_destroy(pbf_ctx* c) {
+  Ctx* x = C(c);
+  if (!x) return PBF_OK;
+  (void)hipSetDevice(x->device);
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  free_device(x);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return PBF_OK;
+}
+
+PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (!xyz || n < 1) return fail(x, PBF_ERR_INVALID, "xyz must be non-NULL and n >= 1");
+  if (n >= (int64_t)1 << 31) return fail(x, PBF_ERR_INVALID, "more than 2^31 particles");
+  if (x->nscenes == 1) {
+    x->counts[0] = n;
+  } else {
+    int64_t tot = 0;
+    for (auto k : x->counts) tot += k;
+    if (tot != n)
+      return fail(x, PBF_ERR_INVALID,
+                  "n = " + std::to_string(n) + " but the scenes hold " + std::to_string(tot));
+  }
+  // Validate every particle against its scene box.
+  std::vector<uint32_t> slot_scene((size_t)n);
+  for (int s = 0; s < x->nscenes; s++) {
+    const SceneGrid& g = x->grids[s];
+    for (int64_t i = x->offsets[s]; i < x->offsets[s] + x->counts[s]; i++) {
+      slot_scene[(size_t)i] = (uint32_t)s;
+      for (int a = 0; a < 3; a++) {
+        const float p = xyz[3 * i + a];
+        const float q = v ? v[3 * i + a] : 0.0f;
+        if (!is_finite_f(p) || !is_finite_f(q))
+          return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(i) + " is not finite");
+        if (p < g.lo[a] || p > g.hi[a])
+          return fail(x, PBF_ERR_OUT_OF_BOX,
+                      "particle " + std::to_string(i) + " lies outside the collision box");
+      }
+    }
+  }
+  HIPCHK(x, hipSetDevice(x->device));
+  x->n = n;
+  x->npad = (n + 63) / 64 * 64;
+  int rc = ensure_device(x);
+  if (rc != PBF_OK) return rc;
+  std::vector<float4> hp((size_t)n), hv((size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t id = (uint32_t)i;
+    float idf;
+    memcpy(&idf, &id, 4);
+    hp[(size_t)i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
+    hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0f)
+                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  pbf::DevState& d = x->d;
+  HIPCHK(x, hipMemcpyAsync(d.posA, hp.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
+                           x->stream));
+  HIPCHK(x, hipMemcpyAsync(d.velA, hv.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
+                           x->stream));
+  HIPCHK(x, hipMemcpyAsync(d.grids, x->grids.data(), x->grids.size() * sizeof(SceneGrid),
+                           hipMemcpyHostToDevice, x->stream));
+  HIPCHK(x, hipMemcpyAsync(d.slotScene, slot_scene.data(), (size_t)n * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  x->sim_time = 0.0;
+  x->loaded = true;
+  x->debug_dirty = false;
+  x->has_nbrs = false;
+  return PBF_OK;
+}
+
+PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  if (!is_finite_f(dt) || !(dt > 0.0f) || nsteps < 0)
+    return fail(x, PBF_ERR_INVALID, "dt must be finite and > 0, nsteps >= 0");
+  SimConst k;
+  pbf::derive_constants(x->p, dt, &k);
+  HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  for (int s = 0; s < nsteps; s++) {
+    rc = run_step(x, k, 0);
+    if (rc != PBF_OK) return rc;
+    x->sim_time += (double)dt;
+  }
+  uint32_t over = 0;
+  rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
+  if (rc != PBF_OK) return rc;
+  if (over > 0)
+    return fail(x, PBF_ERR_OVERFLOW,
+                "a particle had " + std::to_string(over) + " neighbours, max_neighbors is " +
+                    std::to_string(x->p.max_neighbors));
+  return PBF_OK;
+}
+
+PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_ms,
+                              int32_t* pass_launches, int32_t npass) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  if (!is_finite_f(dt) || !(dt > 0.0f) || nsteps < 0 || !pass_ms || npass < PBF_NPASS)
+    return fail(x, PBF_ERR_INVALID, "dt > 0, nsteps >= 0, pass_ms[PBF_NPASS] required");
+  SimConst k;
+  pbf::derive_constants(x->p, dt, &k);
+  for (int i = 0; i < npass; i++) {
+    pass_ms[i] = 0.0f;
+    if (pass_launches) pass_launches[i] = 0;
+  }
+  HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  PassEvents pe;
+  for (int s = 0; s < nsteps && rc == PBF_OK; s++) {
+    pe.used = 0;
+    rc = run_step(x, k, 0, &pe);
+    if (rc == PBF_OK) {
+      hipError_t e = hipStreamSynchronize(x->stream);
+      if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
+    }
+    for (int l = 0; l < pe.used && rc == PBF_OK; l++) {
+      float ms = 0.0f;
+      hipError_t e = hipEventElapsedTime(&ms, pe.ev[(size_t)l * 2], pe.ev[(size_t)l * 2 + 1]);
+      if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
+      pass_ms[pe.pass[(size_t)l]] += ms;
+      if (pass_launches) pass_launches[pe.pass[(size_t)l]] += 1;
+    }
+    if (rc == PBF_OK) x->sim_time += (double)dt;
+  }
+  for (hipEvent_t e : pe.ev) (void)hipEventDestroy(e);
+  if (rc != PBF_OK) return rc;
+  uint32_t over = 0;
+  rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
+  if (rc != PBF_OK) return rc;
+  if (over > 0)
+    return fail(x, PBF_ERR_OVERFLOW,
+                "a particle had " + std::to_string(over) + " neighbours, max_neighbors is " +
+                    std::to_string(x->p.max_neighbors));
+  return PBF_OK;
+}
+
+PBF_API int pbf_step(pbf_ctx* c, int32_t nsteps) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  return pbf_step_dt(c, x->p.dt, nsteps);
+}
+
+PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  if (!is_finite_f(dt) || !(dt > 0.0f) || stage < PBF_STAGE_PREDICT || stage > PBF_STAGE_XSPH)
+    return fail(x, PBF_ERR_INVALID, "dt must be > 0 and stage a PBF_STAGE_* value");
+  SimConst k;
+  pbf::derive_constants(x->p, dt, &k);
+  HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  x->debug_dirty = true;
+  rc = run_step(x, k, stage);
+  if (rc != PBF_OK) return rc;
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  return PBF_OK;
+}
+
+PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  const size_t n = (size_t)x->n;
+  pbf::DevState& d = x->d;
+  if (xyz) {
+    HIPCHK(x, pbf::launch_unpermute(*x, d.posA, d.posA, 0, 3, d.stage));
+    if ((rc = copy_out(x, d.stage, xyz, n * 3 * sizeof(float))) != PBF_OK) return rc;
+  }
+  if (v) {
+    HIPCHK(x, pbf::launch_unpermute(*x, d.posA, d.velA, 0, 3, d.stage));
+    if ((rc = copy_out(x, d.stage, v, n * 3 * sizeof(float))) != PBF_OK) return rc;
+  }
+  if (rho) {
+    HIPCHK(x, pbf::launch_unpermute(*x, d.posA, d.velA, 3, 1, d.stage));
+    if ((rc = copy_out(x, d.stage, rho, n * sizeof(float))) != PBF_OK) return rc;
+  }
+  return PBF_OK;
+}
+
+PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!rho) return fail(x, PBF_ERR_INVALID, "rho is NULL");
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  SimConst k;
+  pbf::derive_constants(x->p, x->p.dt, &k);
+  HIPCHK(x, pbf::launch_densities(*x, k, x->d.stage));
+  if ((rc = copy_out(x, x->d.stage, rho, (size_t)x->n * sizeof(float))) != PBF_OK) return rc;
+  // The density pass re-sorted the committed state into posB/velB.
+  std::swap(x->d.posA, x->d.posB);
+  std::swap(x->d.velA, x->d.velB);
+  x->has_nbrs = false;
+  return PBF_OK;
+}
+
+PBF_API double pbf_simulate_time(const pbf_ctx* c) {
+  const Ctx* x = C(c);
+  return x ? x->sim_time : 0.0;
+}
+
+PBF_API int64_t pbf_count(const pbf_ctx* c) {
+  const Ctx* x = C(c);
+  return x ? x->n : 0;
+}
+
+PBF_API const char* pbf_last_error(const pbf_ctx* c) {
+  const Ctx* x = C(c);
+  if (x && !x->err.empty()) return x->err.c_str();
+  return g_err.c_str();
+}
+
+PBF_API int pbf_neighbors(pbf_ctx* c, uint32_t* counts, uint32_t* ids, int32_t stride) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!x->has_nbrs) return fail(x, PBF_ERR_STATE, "no neighbour lists: run a step first");
+  if (!counts || !ids || stride < 1) return fail(x, PBF_ERR_INVALID, "counts/ids/stride");
+  const size_t n = (size_t)x->n;
+  uint32_t *dc = nullptr, *di = nullptr;
+  HIPCHK(x, hipMalloc((void**)&dc, n * sizeof(uint32_t)));
+  hipError_t e = hipMalloc((void**)&di, n * (size_t)stride * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(dc);
+    return fail(x, PBF_ERR_NOMEM, "neighbour export buffer");
+  }
+  e = pbf::launch_nbr_export(*x, dc, di, stride);
+  if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * sizeof(uint32_t), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ids, di, n * (size_t)stride * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  (void)hipFree(dc);
+  (void)hipFree(di);
+  if (e != hipSuccess) return fail(x, PBF_ERR_HIP, std::string("neighbour export: ") + hipGetErrorString(e));
+  return PBF_OK;
+}
+
+PBF_API int pbf_neighbor_stats(pbf_ctx* c, uint64_t* total, uint32_t* max_count) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!x->has_nbrs) return fail(x, PBF_ERR_STATE, "no neighbour lists: run a step first");
+  if (!total || !max_count) return fail(x, PBF_ERR_INVALID, "total/max_count");
+  // flags[2..3] hold the 64-bit sum, flags[4] the maximum.
+  uint32_t* f = x->d.flags;
+  HIPCHK(x, pbf::launch_fill_u32(f + 2, 3, 0u, x->stream));
+  HIPCHK(x, pbf::launch_nbr_stats(*x, reinterpret_cast<unsigned long long*>(f + 2), f + 4));
+  uint32_t h[3];
+  if ((rc = copy_out(x, reinterpret_cast<const float*>(f + 2), h, sizeof h)) != PBF_OK) return rc;
+  *total = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
+  *max_count = h[2];
+  return PBF_OK;
+}
+
+PBF_API int pbf_cells(pbf_ctx* c, uint32_t* cells) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!x->has_nbrs) return fail(x, PBF_ERR_STATE, "no grid: run a step first");
+  if (!cells) return fail(x, PBF_ERR_INVALID, "cells is NULL");
+  uint32_t* dst = reinterpret_cast<uint32_t*>(x->d.stage);
+  HIPCHK(x, pbf::launch_unpermute_u32(*x, x->d.posB, x->d.cellSorted, dst));
+  return copy_out(x, x->d.stage, cells, (size_t)x->n * sizeof(uint32_t));
+}
+
+PBF_API int pbf_grid_dims(const pbf_ctx* c, int32_t scene, int32_t* dims) {
+  const Ctx* x = C(c);
+  if (!x || !dims || scene < 0 || scene >= x->nscenes)
+    return fail(const_cast<Ctx*>(x), PBF_ERR_INVALID, "bad scene or dims");
+  for (int a = 0; a < 3; a++) dims[a] = x->grids[(size_t)scene].dims[a];
+  return PBF_OK;
+}
+
+PBF_API int pbf_debug_read(pbf_ctx* c, int32_t which, float* out4) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!out4) return fail(x, PBF_ERR_INVALID, "out4 is NULL");
+  pbf::DevState& d = x->d;
+  const float4* src = nullptr;
+  const float4* ids = x->debug_dirty ? d.posB : d.posA;
+  switch (which) {
+    case PBF_BUF_PRED: src = x->dbg_pred; break;
+    case PBF_BUF_VEL: src = x->debug_dirty ? x->dbg_vel : d.velA; break;
+    case PBF_BUF_VXSPH: src = d.vxsph; break;
+    case PBF_BUF_OMEGA: src = d.vort; break;
+    case PBF_BUF_POS: src = d.posA; ids = d.posA; break;
+    default: return fail(x, PBF_ERR_INVALID, "unknown buffer");
+  }
+  if (!src) return fail(x, PBF_ERR_STATE, "buffer not produced yet");
+  HIPCHK(x, pbf::launch_unpermute(*x, ids, src, 0, 4, d.stage));
+  return copy_out(x, d.stage, out4, (size_t)x->n * 4 * sizeof(float));
+}
+
+PBF_API int pbf_debug_derived(const pbf_ctx* c, float* out, int32_t cap) {
+  const Ctx* x = C(c);
+  if (!x || !out || cap < 25) return fail(const_cast<Ctx*>(x), PBF_ERR_INVALID, "cap >= 25");
+  SimConst k;
+  pbf::derive_constants(x->p, x->p.dt, &k);
+  const SceneGrid& g = x->grids[0];
+  const float vals[25] = {k.h,   k.h2,   k.h6,     k.h9,   k.c1,  k.m3c2, k.S,
+                          k.inv_rho0, k.rho0, k.eps, k.negk, k.xc, k.dtv,  k.inv_dt,
+                          k.dt,  k.gdt,  x->p.h * x->p.cell_scale, g.inv_cell, k.eps_d,
+                          g.lo_e[0], g.lo_e[1], g.lo_e[2], g.hi_e[0], g.hi_e[1], g.hi_e[2]};
+  memcpy(out, vals, sizeof vals);
+  return 25;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// pbf_device.h -- device-side arithmetic of the PBF step for gfx950.
+//
+// Every expression keeps the operation order of the reference's FP64 code
+// (src/particles.cpp:51-248 on CGL::Vector3D, CGL/include/CGL/vector3D.h):
+// the build compiles with -ffp-contract=off and correctly rounded division
+// and square root, so that the FP32 results are reproducible bit for bit by
+// a CPU that evaluates the same expressions in IEEE single precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pbf {
+
+// Step constants, derived on the host once per (params, dt).
+struct SimConst {
+  float h, h2, h6, h9;   // H, H*H, intpow<6>(H), intpow<9>(H)
+  float c1;              // poly6 literal 1.56668147106
+  float m3c2;            // -3 * 4.774648292756860
+  float S;               // tensile_instability_scale (particles.cpp:151)
+  float inv_rho0, rho0;  // 1/rho0, rho0
+  float eps;             // EPSILON
+  float negk;            // -K
+  float xc;              // XSPH C
+  float dtv;             // dt * VORTICITY_EPSILON
+  float inv_dt, dt;      // 1/dt, dt
+  float gdt;             // g * dt
+  float eps_d;           // EPS_D
+  int n_tensile;         // N
+};
+
+// Grid and collision box of one scene.
+struct SceneGrid {
+  float lo[3], hi[3];      // box
+  float lo_e[3], hi_e[3];  // lo + EPS_D, hi - EPS_D (the final per-axis clamp)
+  float inv_cell;          // 1 / (h * cell_scale)
+  int dims[3];             // nx, ny, nz
+  uint32_t cell_base;      // first global cell of this scene
+  uint32_t stop;           // analytic stop-plane walls, bit 2*axis+side
+};
+
+// poly6_kernel (src/particles.cpp:134-141).
+__device__ __forceinline__ float poly6(const SimConst& c, float r2) {
+  if (r2 >= c.h2) return 0.0f;
+  float t = c.h2 - r2;
+  return (c.c1 * ((t * t) * t)) / c.h9;
+}
+
+// grad_spiky_kernel (src/particles.cpp:143-149); r2 = (rx*rx+ry*ry)+rz*rz.
+__device__ __forceinline__ float3 spiky(const SimConst& c, float rx, float ry, float rz, float r2) {
+  float rl = sqrtf(r2);
+  if (rl >= c.h || rl < c.eps_d) return make_float3(0.0f, 0.0f, 0.0f);
+  float q = c.h - rl;
+  float coef = c.m3c2 * (q * q);
+  float rc = 1.0f / (c.h6 * rl);
+  return make_float3(rc * (coef * rx), rc * (coef * ry), rc * (coef * rz));
+}
+
+// intpow<n>(b) = ((b*b)*b)... (src/bsdf.h:49-57).
+__device__ __forceinline__ float ipow(float b, int n) {
+  if (n <= 0) return 1.0f;
+  float r = b;
+  for (int k = 1; k < n; k++) r = r * b;
+  return r;
+}
+
+// std::min / std::max as the reference's per-axis clamp uses them.
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ float comp(const float3& v, int a) {
+  return a == 0 ? v.x : (a == 1 ? v.y : v.z);
+}
+
+// Box collision along dir for a displacement D of length l (= total).
+// Analytic stop planes first (strict 0 < t < l, as src/particles.cpp:59-74),
+// then the nearest mesh wall (0 <= t <= l, as the BVH triangle test); on a hit
+// move to t - EPS_D, and in prediction slide once along a mesh wall for half
+// the remaining length (:114-124).  Ends with the per-axis clamp (:129-131).
+__device__ __forceinline__ float3 collide(const SceneGrid& g, const SimConst& c, float3 p,
+                                          float3 D, float3 dir, float l, float total,
+                                          bool slide) {
+  float pa[3] = {p.x, p.y, p.z};
+  const float da[3] = {dir.x, dir.y, dir.z};
+  bool intersect = false;
+#pragma unroll
+  for (int a = 2; a >= 0; a--) {
+    if (da[a] == 0.0f) continue;
+    int hw = da[a] > 0.0f;
+    if (!((g.stop >> (2 * a + hw)) & 1u)) continue;
+    float plane = hw ? g.hi[a] : g.lo[a];
+    float pt = (plane - pa[a]) / da[a];
+    if (pt > 0.0f && pt < l) {
+      l = pt;
+      intersect = true;
+    }
+  }
+  int hit = -1;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (da[a] == 0.0f) continue;
+    int hw = da[a] > 0.0f;
+    if ((g.stop >> (2 * a + hw)) & 1u) continue;
+    float plane = hw ? g.hi[a] : g.lo[a];
+    float t = (plane - pa[a]) / da[a];
+    if (t >= 0.0f && t <= l) {
+      l = t;
+      hit = a;
+    }
+  }
+  if (hit >= 0 || intersect) {
+    float s = l - c.eps_d;
+    pa[0] = pa[0] + s * da[0];
+    pa[1] = pa[1] + s * da[1];
+    pa[2] = pa[2] + s * da[2];
+    if (slide && hit >= 0 && !intersect && -fabsf(da[hit]) > -1.0f) {
+      float T[3] = {D.x, D.y, D.z};
+      T[hit] = 0.0f;
+      float tl2 = T[0] * T[0] + T[1] * T[1] + T[2] * T[2];
+      if (tl2 > 0.0f) {
+        float rn = 1.0f / sqrtf(tl2);
+        float d2[3] = {rn * T[0], rn * T[1], rn * T[2]};
+        float l2 = (total - l) * 0.5f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          if (d2[a] == 0.0f) continue;
+          int hw = d2[a] > 0.0f;
+          if ((g.stop >> (2 * a + hw)) & 1u) continue;
+          float plane = hw ? g.hi[a] : g.lo[a];
+          float t = (plane - pa[a]) / d2[a];
+          if (t >= 0.0f && t <= l2) l2 = t;
+        }
+        float s2 = l2 - c.eps_d;
+        pa[0] = pa[0] + s2 * d2[0];
+        pa[1] = pa[1] + s2 * d2[1];
+        pa[2] = pa[2] + s2 * d2[2];
+      }
+    }
+  } else {
+    pa[0] = pa[0] + D.x;
+    pa[1] = pa[1] + D.y;
+    pa[2] = pa[2] + D.z;
+  }
+  return make_float3(smax(g.lo_e[0], smin(g.hi_e[0], pa[0])),
+                     smax(g.lo_e[1], smin(g.hi_e[1], pa[1])),
+                     smax(g.lo_e[2], smin(g.hi_e[2], pa[2])));
+}
+
+// clamp_response (src/particles.cpp:87-132): prediction with slide.
+__device__ __forceinline__ float3 clamp_response(const SceneGrid& g, const SimConst& c, float3 p,
+                                                 float3 v) {
+  float3 D = make_float3(v.x * c.dt, v.y * c.dt, v.z * c.dt);
+  float total = sqrtf(D.x * D.x + D.y * D.y + D.z * D.z);
+  if (total <= c.eps_d) return p;
+  float rl = 1.0f / total;
+  float3 dir = make_float3(rl * D.x, rl * D.y, rl * D.z);
+  return collide(g, c, p, D, dir, total, total, true);
+}
+
+// clamp (src/particles.cpp:51-84): position correction, no slide.
+__device__ __forceinline__ float3 clamp_dp(const SceneGrid& g, const SimConst& c, float3 p,
+                                           float3 dp) {
+  float l = sqrtf(dp.x * dp.x + dp.y * dp.y + dp.z * dp.z);
+  if (l <= c.eps_d) return p;
+  float rl = 1.0f / l;
+  float3 dir = make_float3(rl * dp.x, rl * dp.y, rl * dp.z);
+  return collide(g, c, p, dp, dir, l, l, false);
+}
+
+// floor((x - lo) * inv_cell) clamped to [0, n-1]; NaN maps to 0.
+__device__ __forceinline__ int axis_cell(float x, float lo, float inv, int n) {
+  float f = floorf((x - lo) * inv);
+  if (!(f >= 0.0f)) return 0;
+  if (!(f < (float)n)) return n - 1;
+  return (int)f;
+}
+
+// Tiled ELL neighbour slot: a wave's 64 particles store neighbour k
+// contiguously, and a wave's whole list block is contiguous.
+__device__ __forceinline__ size_t nbr_slot(uint32_t i, int k, int K) {
+  return ((size_t)(i >> 6) * (size_t)K + (size_t)k) * 64u + (i & 63u);
+}
+
+}  // namespace pbf

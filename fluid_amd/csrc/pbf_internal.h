@@ -1,0 +1,92 @@
+// pbf_internal.h -- context layout and kernel launchers shared by the C ABI
+// (pbf_capi.hip) and the kernels (pbf_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pbf.h"
+#include "pbf_device.h"
+
+namespace pbf {
+
+// Device state of one context.  All particle arrays are float4 and are kept
+// in cell-sorted order between steps; posA.w carries the particle's load
+// index (its id) as integer bits.
+struct DevState {
+  float4* posA = nullptr;   // committed x (xyz), id (w)
+  float4* posB = nullptr;   // sorted copy of x for the current step
+  float4* velA = nullptr;   // committed v (xyz), density (w)
+  float4* velB = nullptr;   // sorted v for the current step
+  float4* predP = nullptr;  // x* ping (xyz), lambda or |omega| (w)
+  float4* predQ = nullptr;  // x* pong
+  float4* vort = nullptr;   // omega (xyz)
+  float4* vxsph = nullptr;  // v after XSPH (xyz), density (w)
+  uint32_t* cell = nullptr;       // cell of particle, unsorted
+  uint32_t* arrival = nullptr;    // arrival rank within its cell
+  uint32_t* cellSorted = nullptr; // cell of particle, sorted
+  uint32_t* tmpId = nullptr;      // ids scattered by cell, unordered within a cell
+  uint32_t* cellCount = nullptr;  // C
+  uint32_t* cellStart = nullptr;  // C + 1, exclusive prefix
+  uint32_t* blockSums = nullptr;  // scan partials
+  uint32_t* nbr = nullptr;        // tiled ELL neighbour lists, npad * K
+  uint32_t* ncnt = nullptr;       // neighbour counts
+  uint32_t* flags = nullptr;      // [0] max neighbour count seen beyond K
+  SceneGrid* grids = nullptr;     // per-scene grids (multi-scene)
+  uint32_t* slotScene = nullptr;  // scene of each sorted slot (multi-scene)
+  float* stage = nullptr;         // read-back staging, n * 4 floats
+};
+
+struct Ctx {
+  pbf_params p{};
+  int nscenes = 1;
+  std::vector<int64_t> counts;    // particles per scene
+  std::vector<int64_t> offsets;   // first particle of each scene
+  std::vector<SceneGrid> grids;   // host copy
+  int64_t n = 0;                  // total particles
+  int64_t npad = 0;               // n rounded up to 64
+  uint32_t ncell = 0;             // total cells
+  uint32_t nblocks_scan = 0;
+  int block = 256;                // threads per block of the particle kernels
+  hipStream_t stream = nullptr;
+  int device = 0;
+  DevState d;
+  size_t alloc_n = 0, alloc_cells = 0;
+  double sim_time = 0.0;
+  bool loaded = false;
+  bool debug_dirty = false;
+  bool has_nbrs = false;
+  const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop
+  const float4* dbg_vel = nullptr;   // velocity buffer current at the last debug stop
+  std::string err;
+};
+
+// Host-side derivation of the step constants, in float, in the order the
+// reference evaluates them.
+void derive_constants(const pbf_params& p, float dt, SimConst* c);
+void derive_grid(const pbf_params& p, const float* lo, const float* hi, SceneGrid* g);
+
+// Kernel launchers (pbf_kernels.hip).  Each returns hipSuccess or the launch
+// error.
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+hipError_t launch_predict(Ctx& x, const SimConst& c);
+hipError_t launch_scan(Ctx& x);
+hipError_t launch_scatter_rank(Ctx& x);
+hipError_t launch_nbr_build(Ctx& x, const SimConst& c);
+hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out);
+hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last);
+hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out);
+hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in);
+hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int first, int count,
+                            float* dst);
+hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, uint32_t* dst);
+hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride);
+hipError_t launch_densities(Ctx& x, const SimConst& c, float* out);
+hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx);
+hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
+                           hipStream_t s);
+
+}  // namespace pbf

@@ -1,0 +1,187 @@
+/*
+ * pbf.h -- C ABI of the MI355X Position Based Fluids step.
+ *
+ * This is the drop-in boundary underneath the reference's `Particles`
+ * load/step interface (src/particles.h:105-140).  Plain C types only: host
+ * pointers to float arrays, sizes, and an opaque context.  All entry points
+ * return PBF_OK (0) or a negative PBF_ERR_* code and never exit; the message
+ * of the last failure is available from pbf_last_error().
+ *
+ * Arrays exchanged with the host are in load order ("particle id" order):
+ * particle i of pbf_load is entry i of every read-back.  Positions and
+ * velocities are n*3 floats (x, y, z interleaved).
+ *
+ * One context is not re-entrant.  Every entry point is synchronous at
+ * return (work runs on the context's own HIP stream).
+ */
+#ifndef PBF_H
+#define PBF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_API __attribute__((visibility("default")))
+
+enum {
+  PBF_OK = 0,
+  PBF_ERR_INVALID = -1,   /* bad argument (NULL, n = 0, non-finite, dt <= 0 ...) */
+  PBF_ERR_HIP = -2,       /* HIP runtime failure or no device */
+  PBF_ERR_OVERFLOW = -3,  /* a particle had more neighbours than max_neighbors */
+  PBF_ERR_STATE = -4,     /* call out of order (e.g. step before load) */
+  PBF_ERR_OUT_OF_BOX = -5,/* a loaded particle lies outside the collision box */
+  PBF_ERR_NOMEM = -6      /* device or host allocation failed */
+};
+
+/* Physical and numerical parameters.  The reference fixes these as
+ * compile-time macros (src/particles.cpp:7-48); here they are run-time
+ * values.  pbf_params_default() fills the reference's own values. */
+typedef struct pbf_params {
+  float h;               /* smoothing radius H (particles.cpp:26) */
+  float rest_density;    /* rho0 (Particles(double rest_density), particles.h:114) */
+  int32_t iterations;    /* solver iterations NEWTON_NUM_STEPS (:34) */
+  float epsilon;         /* constraint relaxation EPSILON (:36) */
+  float k_tensile;       /* artificial pressure K (:38) */
+  int32_t n_tensile;     /* artificial pressure exponent N (:40) */
+  float dq;              /* tensile reference distance / h (:151, 0.1) */
+  float xsph_c;          /* XSPH viscosity C (:44) */
+  float vort_eps;        /* vorticity confinement VORTICITY_EPSILON (:42) */
+  float gravity;         /* acceleration along -y (:180, 10) */
+  float dt;              /* default time step DEFAULT_DELTA_T (:24) */
+  float box_lo[3];       /* collision box, lower corner */
+  float box_hi[3];       /* collision box, upper corner */
+  uint32_t wall_stop_mask; /* bit (2*axis + side): that wall is an analytic
+                              stop plane (no slide, strict crossing); the
+                              other walls behave as the reference's mesh walls */
+  float cell_scale;      /* grid cell edge = h * cell_scale, >= 1 */
+  int32_t max_neighbors; /* neighbour-list capacity per particle */
+  float poly6_coef;      /* 315/(64 pi) as the reference literal (:140) */
+  float spiky_coef;      /* 15/pi as the reference literal (:148) */
+  float eps_d;           /* EPS_D (CGL/include/CGL/misc.h:11) */
+} pbf_params;
+
+typedef struct pbf_ctx pbf_ctx;
+
+/* Fills the reference's constants: H=0.3, 12 iterations, EPSILON=2, K=1e-4,
+ * N=4, dq=0.1, C=1e-3, VORTICITY_EPSILON=1e-3, g=10, dt=0.016, the Cornell
+ * box [-1,1]x[0,1.49]x[-1,1] with stop planes y=1.49 and z=1
+ * (src/particles.cpp:7-48, :59-83). */
+PBF_API void pbf_params_default(pbf_params *p);
+
+/* Creates a context on the current HIP device.  Replaces the reference's
+ * constructor Particles(double rest_density) (src/particles.h:114). */
+PBF_API int pbf_create(const pbf_params *p, pbf_ctx **out);
+
+/* Multi-scene context: nscenes independent scenes with counts[s] particles
+ * and boxes box_lo/box_hi[3*s..3*s+2]; all scenes share the physics of p.
+ * Loaded state is the concatenation of the scenes in order.  No counterpart
+ * in the reference (batched extension). */
+PBF_API int pbf_create_batch(const pbf_params *p, int32_t nscenes, const int64_t *counts,
+                             const float *box_lo, const float *box_hi, pbf_ctx **out);
+
+PBF_API int pbf_destroy(pbf_ctx *ctx);
+
+/* Uploads n particles (positions xyz, velocities v, each n*3 floats; v may
+ * be NULL for zero).  Replaces the reference's repeated addParticle
+ * (src/particles.h:118-120, called from src/application.cpp:334-340).
+ * Resets simulate_time to 0. */
+PBF_API int pbf_load(pbf_ctx *ctx, const float *xyz, const float *v, int64_t n);
+
+/* Advances nsteps steps of length dt.  Replaces Particles::timeStep(double)
+ * (src/particles.cpp:250-297). */
+PBF_API int pbf_step_dt(pbf_ctx *ctx, float dt, int32_t nsteps);
+
+/* Advances nsteps steps of the default dt.  Replaces Particles::timeStep()
+ * (src/particles.cpp:299-301). */
+PBF_API int pbf_step(pbf_ctx *ctx, int32_t nsteps);
+
+/* Copies the committed state back to the host in load order; any pointer
+ * may be NULL.  rho is the SPH density of the last step's XSPH pass (the
+ * value the reference keeps in Particle::density, src/particles.cpp:222-231).
+ * Replaces the reads of ps[i]->getPosition() / velocity / density by the
+ * CPU consumers (src/particles.cpp:305, :449, src/particles.h:36-52). */
+PBF_API int pbf_read(pbf_ctx *ctx, float *xyz, float *v, float *rho);
+
+/* Load-time density including self, the reference's
+ * Particles::estimateDensities (src/particles.cpp:440-444 -> :158-163),
+ * evaluated on the grid.  Writes n floats in load order. */
+PBF_API int pbf_estimate_densities(pbf_ctx *ctx, float *rho);
+
+/* Particle::simulate_time of the reference (src/particles.h:109). */
+PBF_API double pbf_simulate_time(const pbf_ctx *ctx);
+PBF_API int64_t pbf_count(const pbf_ctx *ctx);
+
+/* Message of the last failure on ctx (or of the last failed pbf_create*
+ * when ctx is NULL).  Never NULL. */
+PBF_API const char *pbf_last_error(const pbf_ctx *ctx);
+
+/* ---- test hooks (inspection of the last step; no counterpart) ---------- */
+
+/* Neighbour lists built by the last step, in load order: counts[n]; ids
+ * receives, for particle i, its neighbours' load-order ids in the order the
+ * passes visit them, at ids[i*stride .. i*stride+counts[i]).  Neighbour sets
+ * are those of src/particles.cpp:258-265. */
+PBF_API int pbf_neighbors(pbf_ctx *ctx, uint32_t *counts, uint32_t *ids, int32_t stride);
+
+/* Sum and maximum of the neighbour counts of the last step (measurement). */
+PBF_API int pbf_neighbor_stats(pbf_ctx *ctx, uint64_t *total, uint32_t *max_count);
+
+/* Grid cell index of each particle at the last neighbour build, load order. */
+PBF_API int pbf_cells(pbf_ctx *ctx, uint32_t *cells);
+
+/* Grid dimensions of scene s (nx, ny, nz). */
+PBF_API int pbf_grid_dims(const pbf_ctx *ctx, int32_t scene, int32_t *dims);
+
+/* Runs one step but stops after `stage` (see PBF_STAGE_*), leaving the
+ * intermediate buffers readable through pbf_debug_read.  The context must be
+ * reloaded before the next pbf_step. */
+enum {
+  PBF_STAGE_PREDICT = 1,   /* x* and v after applyForceVelocity */
+  PBF_STAGE_NEIGHBORS = 2, /* + neighbour lists */
+  PBF_STAGE_LAMBDA0 = 3,   /* + lambda of iteration 0 */
+  PBF_STAGE_DELTAP0 = 4,   /* + x* after iteration 0 */
+  PBF_STAGE_SOLVE = 5,     /* + all iterations, v = (x* - x)/dt */
+  PBF_STAGE_XSPH = 6       /* + XSPH velocity, vorticity, density */
+};
+PBF_API int pbf_step_debug(pbf_ctx *ctx, float dt, int32_t stage);
+
+enum {
+  PBF_BUF_PRED = 1,   /* x* (xyz) and lambda (w) */
+  PBF_BUF_VEL = 2,    /* current velocity (xyz) */
+  PBF_BUF_VXSPH = 3,  /* velocity after XSPH (xyz), density (w) */
+  PBF_BUF_OMEGA = 4,  /* vorticity (xyz) */
+  PBF_BUF_POS = 5     /* committed position (xyz) */
+};
+/* Reads buffer `which` as n*4 floats in load order. */
+PBF_API int pbf_debug_read(pbf_ctx *ctx, int32_t which, float *out4);
+
+/* Passes of one step, for pbf_step_profiled. */
+enum {
+  PBF_PASS_PREDICT = 0,   /* gravity, prediction + collision, cell hash, histogram */
+  PBF_PASS_SCAN = 1,      /* exclusive scan of the cell histogram */
+  PBF_PASS_SORT = 2,      /* counting-sort scatter + in-cell rank + reorder */
+  PBF_PASS_NEIGHBORS = 3, /* neighbour-list build */
+  PBF_PASS_LAMBDA = 4,    /* density / lambda (one launch per iteration) */
+  PBF_PASS_DELTAP = 5,    /* delta-p + collision (one launch per iteration) */
+  PBF_PASS_XSPH = 6,      /* XSPH viscosity + vorticity */
+  PBF_PASS_VORTICITY = 7, /* vorticity confinement + commit */
+  PBF_NPASS = 8
+};
+/* Runs nsteps steps like pbf_step_dt, timing every pass with HIP events on
+ * the context stream; pass_ms[p] receives the summed milliseconds of pass p
+ * and pass_launches[p] (may be NULL) its number of launches.  Measurement
+ * hook: the state advances exactly as with pbf_step_dt. */
+PBF_API int pbf_step_profiled(pbf_ctx *ctx, float dt, int32_t nsteps, float *pass_ms,
+                              int32_t *pass_launches, int32_t npass);
+
+/* Derived constants in float, same order as the oracle's ORC_* indices
+ * (h, h2, h^6, h^9, ...), and the number written. */
+PBF_API int pbf_debug_derived(const pbf_ctx *ctx, float *out, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
