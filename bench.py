@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X PBF step (one GPU).
+
+Default workload: the 4,096,000-particle two-sphere collision (h = 0.02031,
+4 solver iterations, dt 0.005).  A step is one full Particles::timeStep
+(src/particles.cpp:250-297) over all particles, with the state resident in
+HBM.  Prints ONE JSON line:
+
+* value      particle-steps/s = N * steps / wall time of the timed steps
+             (pbf_step returns after the device finished);
+* roofline   the dominant pass, from HIP events recorded on the context
+             stream: algorithmic bytes per launch / mean launch time,
+             against 8 TB/s; traffic from the committed rocprofv3 counters;
+* passes     ms per step, GB/s (state-only and state+list) of every pass;
+* cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
+             every host core) timed on a few steps of the same scene.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FP32 state bytes per particle of every pass (reads + writes of
+# the particle state the pass needs; neighbour gathers assumed on-chip).
+STATE_BYTES = {
+    "predict": 64,    # x, v in; x*, v, cell out (+ histogram atomic)
+    "scan": 0,        # 8 bytes per grid cell, added below
+    "sort": 84,       # cell, rank, x, x*, v in; ids, x, x*, v out
+    "neighbors": 16,  # x* in (+ the list it writes, state+list)
+    "lambda": 16,     # x* in, lambda out
+    "deltap": 28,     # x*, lambda in, x*' out
+    "xsph": 48,       # x, x* in (v derived); v, omega out
+    "vorticity": 60,  # x*, v, omega in; v, x out
+}
+LIST_PASSES = ("neighbors", "lambda", "deltap", "xsph", "vorticity")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), committed as profiles/<workload>_traffic.json."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(sc, steps: int):
+    from oracle import oracle
+
+    threads = len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+    log(f"cpu baseline: oracle grid/Jacobi FP32, {threads} threads, {steps} steps of {sc.name}")
+    t0 = time.perf_counter()
+    oracle.step(sc.params, sc.x, sc.v, steps)
+    dt = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {
+        "value": sc.n * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sc.name}: {steps} steps from the initial state, oracle/ restatement "
+                  f"(FP32, uniform grid, Jacobi XSPH, OpenMP), {dt:.2f} s; host {model}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a plain 256 MiB device copy (reference for the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "path is not part of this build", file=sys.stderr)
+        return 2
+    if args.calibrate:
+        import torch
+
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 256 MiB read + 256 MiB write")
+
+    from fluid_amd import scenes
+    from fluid_amd.pbf import Particles
+
+    t0 = time.perf_counter()
+    sc = scenes.two_spheres()
+    log(f"scene {sc.name}: {sc.n} particles, h={sc.params.h}, generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+    ctx = Particles(sc.params)
+    ctx.load(sc.x, sc.v)
+    ctx.time_step(nsteps=args.warmup)
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    ctx.time_step(nsteps=args.steps)
+    wall = time.perf_counter() - t0
+    value = sc.n * args.steps / wall
+    log(f"timed {args.steps} steps: {wall:.3f} s, {value:.3e} particle-steps/s")
+
+    prof = ctx.time_step_profiled(nsteps=args.profile_steps)
+    nbar, nmax = ctx.neighbor_stats()
+    ctx.close()
+    ps = args.profile_steps
+    ncell = 1
+    for a in range(3):
+        ncell *= int((sc.params.box_hi[a] - sc.params.box_lo[a]) / (sc.params.h * sc.params.cell_scale)) + 1
+    passes = {}
+    for name, (ms, launches) in prof.items():
+        if launches == 0:
+            continue
+        per_launch_s = ms / launches * 1e-3
+        state = STATE_BYTES[name] * sc.n + (8 * ncell if name == "scan" else 0)
+        listb = (4.0 * nbar + 4.0) * sc.n if name in LIST_PASSES else 0.0
+        passes[name] = {
+            "ms_per_step": ms / ps,
+            "launches_per_step": launches / ps,
+            "us_per_launch": per_launch_s * 1e6,
+            "state_GBs": state / per_launch_s / 1e9,
+            "state_list_GBs": (state + listb) / per_launch_s / 1e9,
+        }
+    dom = max(passes, key=lambda k: passes[k]["ms_per_step"])
+    d = passes[dom]
+    per_launch_bytes = STATE_BYTES[dom] * sc.n
+    traffic = load_traffic(sc.name)
+    roof = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": d["state_GBs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": d["state_GBs"] / HBM_PEAK_GBS,
+        "traffic": (traffic or {}).get(dom),
+        "bytes_per_launch": per_launch_bytes,
+        "basis": f"state-only {STATE_BYTES[dom]} B/particle x {sc.n} particles per launch",
+        "state_list_frac": d["state_list_GBs"] / HBM_PEAK_GBS,
+    }
+    out = {
+        "metric": "PBF particle-steps/s @4M",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name,
+            "particles": sc.n,
+            "h": sc.params.h,
+            "iterations": sc.params.iterations,
+            "dt": sc.params.dt,
+            "constants": "scaled",
+            "mean_neighbors": nbar,
+            "max_neighbors": nmax,
+        },
+        "roofline": roof,
+        "passes": passes,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

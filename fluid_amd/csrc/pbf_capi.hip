@@ -101,7 +101,7 @@ int ensure_device(Ctx* x) {
   free_device(x);
   pbf::DevState& d = x->d;
   const size_t n = (size_t)x->n, np = (size_t)x->npad, C = (size_t)x->ncell;
-  const size_t K = (size_t)x->p.max_neighbors;
+  const size_t K = (size_t)pbf::k4_of(*x) * pbf::kChunk;
   const size_t nb = (C + 1023) / 1024 + 1;
   hipError_t e = hipSuccess;
 #define DA(ptr, cnt) \

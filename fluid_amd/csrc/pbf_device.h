@@ -175,10 +175,84 @@ __device__ __forceinline__ int axis_cell(float x, float lo, float inv, int n) {
   return (int)f;
 }
 
-// Tiled ELL neighbour slot: a wave's 64 particles store neighbour k
-// contiguously, and a wave's whole list block is contiguous.
-__device__ __forceinline__ size_t nbr_slot(uint32_t i, int k, int K) {
-  return ((size_t)(i >> 6) * (size_t)K + (size_t)k) * 64u + (i & 63u);
+// Neighbour lists: chunked ELL.  A wave's 64 particles own one contiguous
+// tile of K4 = K/4 chunks; chunk c of a particle holds its neighbours
+// 4c..4c+3 as 16 contiguous bytes, and the 64 lanes' chunks c are adjacent
+// (1 KiB).  A builder writes a whole chunk with one 16-byte store; a reader
+// fetches four indices with one 16-byte load, and the 64 lanes of the wave
+// then read one fully used 1 KiB block.
+constexpr int kChunk = 4;
+
+__device__ __forceinline__ size_t nbr_chunk(uint32_t i, int c, int K4) {
+  return (((size_t)(i >> 6) * (size_t)K4 + (size_t)c) * 64u + (i & 63u)) * kChunk;
+}
+
+__device__ __forceinline__ size_t nbr_slot(uint32_t i, int k, int K4) {
+  return nbr_chunk(i, k >> 2, K4) + (size_t)(k & 3);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remapping the hardware block index so that every XCD runs a
+// contiguous range of logical blocks keeps the cell-sorted neighbourhoods a
+// wave gathers from inside one L2.  Bijective on [0, nb).
+#ifndef PBF_XCD_SWIZZLE
+#define PBF_XCD_SWIZZLE 1
+#endif
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  if (!PBF_XCD_SWIZZLE) return b;
+  const uint32_t q = nb >> 3, r = nb & 7u;
+  const uint32_t x = b & 7u, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// Branch-free forms of poly6 / spiky for the neighbour loops: the same
+// expressions evaluated unconditionally, with the reference's cut-offs
+// applied by selection (identical results, no divergent branches).
+__device__ __forceinline__ float poly6_sel(const SimConst& c, float r2) {
+  const float t = c.h2 - r2;
+  const float w = (c.c1 * ((t * t) * t)) / c.h9;
+  return (r2 >= c.h2) ? 0.0f : w;
+}
+
+__device__ __forceinline__ float3 spiky_sel(const SimConst& c, float rx, float ry, float rz,
+                                            float r2) {
+  const float rl = sqrtf(r2);
+  const float q = c.h - rl;
+  const float coef = c.m3c2 * (q * q);
+  const float rc = 1.0f / (c.h6 * rl);
+  const bool zero = (rl >= c.h) || (rl < c.eps_d);
+  return make_float3(zero ? 0.0f : rc * (coef * rx), zero ? 0.0f : rc * (coef * ry),
+                     zero ? 0.0f : rc * (coef * rz));
+}
+
+// Visits the neighbours of particle i in list order.  Groups of U
+// independent index loads, then U independent gathers, are issued before the
+// U bodies run, so the memory latency of a group overlaps; body(j, data) is
+// still called for k = 0, 1, 2, ... in order, keeping the summation order.
+template <int U, class Load, class Body>
+__device__ __forceinline__ void for_each_neighbor(uint32_t i, int cnt, int K4,
+                                                  const uint32_t* __restrict__ nbr, Load load,
+                                                  Body body) {
+  static_assert(U == 4 || U == 8, "groups of 4 or 8 indices");
+  int k = 0;
+  for (; k + U <= cnt; k += U) {
+    uint32_t j[U];
+    const uint4 a = *reinterpret_cast<const uint4*>(nbr + nbr_slot(i, k, K4));
+    j[0] = a.x; j[1] = a.y; j[2] = a.z; j[3] = a.w;
+    if (U == 8) {
+      const uint4 b = *reinterpret_cast<const uint4*>(nbr + nbr_slot(i, k + 4, K4));
+      j[U - 4] = b.x; j[U - 3] = b.y; j[U - 2] = b.z; j[U - 1] = b.w;
+    }
+    decltype(load(0u)) d[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) d[u] = load(j[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) body(j[u], d[u]);
+  }
+  for (; k < cnt; k++) {
+    const uint32_t j = nbr[nbr_slot(i, k, K4)];
+    body(j, load(j));
+  }
 }
 
 }  // namespace pbf

@@ -64,6 +64,10 @@ struct Ctx {
   std::string err;
 };
 
+// Chunks of 4 neighbour slots per particle: the list capacity is
+// max_neighbors rounded up to a multiple of 4.
+inline int k4_of(const Ctx& x) { return (x.p.max_neighbors + 3) / 4; }
+
 // Host-side derivation of the step constants, in float, in the order the
 // reference evaluates them.
 void derive_constants(const pbf_params& p, float dt, SimConst* c);

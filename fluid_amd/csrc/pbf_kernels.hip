@@ -21,6 +21,15 @@ namespace pbf {
 namespace {
 
 constexpr int kBlock = 256;
+// Neighbour loads in flight per group (build-time tunables).
+#ifndef PBF_UNROLL
+#define PBF_UNROLL 4
+#endif
+#ifndef PBF_UNROLL_XSPH
+#define PBF_UNROLL_XSPH 8
+#endif
+constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
+constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers two float4 per neighbour
 
 __device__ __forceinline__ uint32_t id_of(const float4& p) { return __float_as_uint(p.w); }
 
@@ -204,14 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(
 // Neighbour sets of src/particles.cpp:258-265 (|x*_i - x*_j|^2 <= H2, j != i),
 // found in the 27 cells around i.  For each (dz, dy) the three cells dx =
 // -1..1 are contiguous in sorted order, so the search walks 9 index ranges in
-// ascending order.
+// ascending order.  Found indices collect in a 4-entry register chunk that
+// is stored whole (one 16-byte store) when full.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_nbr_build(
     uint32_t n, const float4* __restrict__ pred, const uint32_t* __restrict__ cellStart,
-    uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ flags, int K,
+    uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ flags, int K4,
     float h2, SceneGrid g0, const SceneGrid* __restrict__ grids,
     const uint32_t* __restrict__ slotScene) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const SceneGrid g = grid_of<MULTI>(i, g0, grids, slotScene);
   const float4 xi = pred[i];
@@ -220,6 +230,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(
   const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
   const int x1 = cc[0] + 1 < g.dims[0] ? cc[0] + 1 : g.dims[0] - 1;
   uint32_t cnt = 0;
+  uint32_t buf[kChunk];
+#pragma unroll
+  for (int u = 0; u < kChunk; u++) buf[u] = 0u;
+  auto flush = [&](int chunk) {
+    if (chunk < K4)
+      *reinterpret_cast<uint4*>(nbr + nbr_chunk(i, chunk, K4)) =
+          make_uint4(buf[0], buf[1], buf[2], buf[3]);
+  };
+  auto test = [&](uint32_t j, const float4& xj) {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    if (d2 <= h2 && j != i) {
+      const uint32_t slot = cnt & 3u;
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) buf[u] = (slot == (uint32_t)u) ? j : buf[u];
+      cnt++;
+      if ((cnt & 3u) == 0u) flush((int)(cnt >> 2) - 1);
+    }
+  };
   for (int dz = -1; dz <= 1; dz++) {
     const int z = cc[2] + dz;
     if (z < 0 || z >= g.dims[2]) continue;
@@ -230,21 +259,22 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(
           g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
       const uint32_t s = cellStart[row + (uint32_t)x0];
       const uint32_t e = min(cellStart[row + (uint32_t)x1 + 1u], n);
-      for (uint32_t j = s; j < e; j++) {
-        if (j == i) continue;
-        const float4 xj = pred[j];
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float d2 = rx * rx + ry * ry + rz * rz;
-        if (d2 <= h2) {
-          if ((int)cnt < K) nbr[nbr_slot(i, (int)cnt, K)] = j;
-          cnt++;
-        }
+      uint32_t j = s;
+      for (; j + 4 <= e; j += 4) {
+        const float4 a = pred[j], b = pred[j + 1], c4 = pred[j + 2], d = pred[j + 3];
+        test(j, a);
+        test(j + 1, b);
+        test(j + 2, c4);
+        test(j + 3, d);
       }
+      for (; j < e; j++) test(j, pred[j]);
     }
   }
-  if ((int)cnt > K) {
+  if (cnt & 3u) flush((int)(cnt >> 2));
+  const uint32_t cap = (uint32_t)K4 * kChunk;
+  if (cnt > cap) {
     atomicMax(&flags[0], cnt);
-    cnt = (uint32_t)K;
+    cnt = cap;
   }
   ncnt[i] = cnt;
 }
@@ -253,29 +283,29 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(
 // newtonStepCalculateLambda (src/particles.cpp:185-204).  out = (x*, lambda).
 __global__ __launch_bounds__(kBlock) void k_lambda(uint32_t n, const float4* __restrict__ in,
                                                    const uint32_t* __restrict__ nbr,
-                                                   const uint32_t* __restrict__ ncnt, int K,
+                                                   const uint32_t* __restrict__ ncnt, int K4,
                                                    SimConst c, float4* __restrict__ out) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 xi = in[i];
   const int cnt = (int)ncnt[i];
   float dens = 0.0f, denom = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t j = nbr[nbr_slot(i, k, K)];
-    const float4 xj = in[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float w = poly6(c, r2);
-    float3 gr = spiky(c, rx, ry, rz, r2);
-    dens = dens + w;
-    gr.x = gr.x * c.inv_rho0;
-    gr.y = gr.y * c.inv_rho0;
-    gr.z = gr.z * c.inv_rho0;
-    Gx = Gx + gr.x;
-    Gy = Gy + gr.y;
-    Gz = Gz + gr.z;
-    denom = denom + (gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
-  }
+  for_each_neighbor<kUnroll>(
+      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      [&](uint32_t, const float4& xj) {
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float w = poly6_sel(c, r2);
+        float3 gr = spiky_sel(c, rx, ry, rz, r2);
+        dens = dens + w;
+        gr.x = gr.x * c.inv_rho0;
+        gr.y = gr.y * c.inv_rho0;
+        gr.z = gr.z * c.inv_rho0;
+        Gx = Gx + gr.x;
+        Gy = Gy + gr.y;
+        Gz = Gz + gr.z;
+        denom = denom + (gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+      });
   const float ci = dens / c.rho0 - 1.0f;
   denom = denom + (Gx * Gx + Gy * Gy + Gz * Gz);
   const float lambda = -ci / (denom + c.eps);
@@ -288,27 +318,27 @@ __global__ __launch_bounds__(kBlock) void k_lambda(uint32_t n, const float4* __r
 template <bool MULTI, bool LAST>
 __global__ __launch_bounds__(kBlock) void k_deltap(
     uint32_t n, const float4* __restrict__ in, const uint32_t* __restrict__ nbr,
-    const uint32_t* __restrict__ ncnt, int K, SimConst c, SceneGrid g0,
+    const uint32_t* __restrict__ ncnt, int K4, SimConst c, SceneGrid g0,
     const SceneGrid* __restrict__ grids, const uint32_t* __restrict__ slotScene,
     float4* __restrict__ out, const float4* __restrict__ pos, float4* __restrict__ vel) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 xi = in[i];
   const int cnt = (int)ncnt[i];
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t j = nbr[nbr_slot(i, k, K)];
-    const float4 xj = in[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float w = poly6(c, r2);
-    const float s = c.negk * ipow(w * c.S, c.n_tensile);
-    const float3 gr = spiky(c, rx, ry, rz, r2);
-    const float sc = (xi.w + xj.w) + s;
-    dx = dx + sc * gr.x;
-    dy = dy + sc * gr.y;
-    dz = dz + sc * gr.z;
-  }
+  for_each_neighbor<kUnroll>(
+      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      [&](uint32_t, const float4& xj) {
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float w = poly6_sel(c, r2);
+        const float s = c.negk * ipow(w * c.S, c.n_tensile);
+        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+        const float sc = (xi.w + xj.w) + s;
+        dx = dx + sc * gr.x;
+        dy = dy + sc * gr.y;
+        dz = dz + sc * gr.z;
+      });
   dx = dx * c.inv_rho0;
   dy = dy * c.inv_rho0;
   dz = dz * c.inv_rho0;
@@ -328,33 +358,35 @@ __global__ __launch_bounds__(kBlock) void k_deltap(
 __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, const float4* __restrict__ in,
                                                  const float4* __restrict__ vel,
                                                  const uint32_t* __restrict__ nbr,
-                                                 const uint32_t* __restrict__ ncnt, int K,
+                                                 const uint32_t* __restrict__ ncnt, int K4,
                                                  SimConst c, float4* __restrict__ out,
                                                  float4* __restrict__ vx,
                                                  float4* __restrict__ vort) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 xi = in[i];
   const float4 vi = vel[i];
   const int cnt = (int)ncnt[i];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, dens = 0.0f;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t j = nbr[nbr_slot(i, k, K)];
-    const float4 vj = vel[j];
-    const float u0 = vj.x - vi.x, u1 = vj.y - vi.y, u2 = vj.z - vi.z;
-    const float4 xj = in[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float3 gr = spiky(c, rx, ry, rz, r2);
-    ox = ox + (u1 * gr.z - u2 * gr.y);
-    oy = oy + (u2 * gr.x - u0 * gr.z);
-    oz = oz + (u0 * gr.y - u1 * gr.x);
-    const float w = poly6(c, r2);
-    sx = sx + u0 * w;
-    sy = sy + u1 * w;
-    sz = sz + u2 * w;
-    dens = dens + w;
-  }
+  struct XV {
+    float4 x, v;
+  };
+  for_each_neighbor<kUnrollXsph>(
+      i, cnt, K4, nbr,[&](uint32_t j) { return XV{in[j], vel[j]}; },
+      [&](uint32_t, const XV& d) {
+        const float u0 = d.v.x - vi.x, u1 = d.v.y - vi.y, u2 = d.v.z - vi.z;
+        const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+        ox = ox + (u1 * gr.z - u2 * gr.y);
+        oy = oy + (u2 * gr.x - u0 * gr.z);
+        oz = oz + (u0 * gr.y - u1 * gr.x);
+        const float w = poly6_sel(c, r2);
+        sx = sx + u0 * w;
+        sy = sy + u1 * w;
+        sz = sz + u2 * w;
+        dens = dens + w;
+      });
   const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
   out[i] = make_float4(xi.x, xi.y, xi.z, wn);
   vx[i] = make_float4(vi.x + c.xc * sx, vi.y + c.xc * sy, vi.z + c.xc * sz, dens);
@@ -366,23 +398,23 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, const float4* __res
 __global__ __launch_bounds__(kBlock) void k_vort_commit(
     uint32_t n, const float4* __restrict__ in, const float4* __restrict__ vort,
     const float4* __restrict__ vx, const float4* __restrict__ posB,
-    const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ ncnt, int K, SimConst c,
+    const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ ncnt, int K4, SimConst c,
     float4* __restrict__ posA, float4* __restrict__ velA) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 xi = in[i];
   const int cnt = (int)ncnt[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t j = nbr[nbr_slot(i, k, K)];
-    const float4 xj = in[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    const float3 gr = spiky(c, rx, ry, rz, r2);
-    ex = ex + xj.w * gr.x;
-    ey = ey + xj.w * gr.y;
-    ez = ez + xj.w * gr.z;
-  }
+  for_each_neighbor<kUnroll>(
+      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      [&](uint32_t, const float4& xj) {
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+        ex = ex + xj.w * gr.x;
+        ey = ey + xj.w * gr.y;
+        ez = ez + xj.w * gr.z;
+      });
   float4 v = vx[i];
   const float e2 = ex * ex + ey * ey + ez * ez;
   if (sqrtf(e2) > c.eps_d) {
@@ -460,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_unpermute_u32(uint32_t n,
 
 __global__ __launch_bounds__(kBlock) void k_nbr_export(uint32_t n, const float4* __restrict__ ids,
                                                        const uint32_t* __restrict__ nbr,
-                                                       const uint32_t* __restrict__ ncnt, int K,
+                                                       const uint32_t* __restrict__ ncnt, int K4,
                                                        uint32_t* __restrict__ counts,
                                                        uint32_t* __restrict__ out, int stride) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -469,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_export(uint32_t n, const float4*
   const int cnt = (int)ncnt[i];
   counts[id] = (uint32_t)cnt;
   const int m = cnt < stride ? cnt : stride;
-  for (int k = 0; k < m; k++) out[(size_t)id * stride + k] = id_of(ids[nbr[nbr_slot(i, k, K)]]);
+  for (int k = 0; k < m; k++) out[(size_t)id * stride + k] = id_of(ids[nbr[nbr_slot(i, k, K4)]]);
 }
 
 // Sum and maximum of the neighbour counts (out[0..1] = sum lo/hi, out[2] = max).
@@ -548,18 +580,18 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
   const unsigned nb = blocks_for(x.n);
   if (multi(x))
     k_nbr_build<true><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.cellStart, d.nbr,
-                                                   d.ncnt, d.flags, x.p.max_neighbors, c.h2,
+                                                   d.ncnt, d.flags, k4_of(x), c.h2,
                                                    x.grids[0], d.grids, d.slotScene);
   else
     k_nbr_build<false><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.cellStart, d.nbr,
-                                                    d.ncnt, d.flags, x.p.max_neighbors, c.h2,
+                                                    d.ncnt, d.flags, k4_of(x), c.h2,
                                                     x.grids[0], d.grids, d.slotScene);
   return hipGetLastError();
 }
 
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out) {
   k_lambda<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, x.d.nbr, x.d.ncnt,
-                                                     x.p.max_neighbors, c, out);
+                                                     k4_of(x), c, out);
   return hipGetLastError();
 }
 
@@ -567,7 +599,7 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
   DevState& d = x.d;
   const unsigned nb = blocks_for(x.n);
   const uint32_t n = (uint32_t)x.n;
-  const int K = x.p.max_neighbors;
+  const int K = k4_of(x);
   if (multi(x)) {
     if (last)
       k_deltap<true, true><<<nb, kBlock, 0, x.stream>>>(n, in, d.nbr, d.ncnt, K, c, x.grids[0],
@@ -592,7 +624,7 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
 hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out) {
   DevState& d = x.d;
   k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, d.velB, d.nbr, d.ncnt,
-                                                   x.p.max_neighbors, c, out, d.vxsph, d.vort);
+                                                   k4_of(x), c, out, d.vxsph, d.vort);
   return hipGetLastError();
 }
 
@@ -600,7 +632,7 @@ hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
   DevState& d = x.d;
   k_vort_commit<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, d.vort, d.vxsph,
                                                           d.posB, d.nbr, d.ncnt,
-                                                          x.p.max_neighbors, c, d.posA, d.velA);
+                                                          k4_of(x), c, d.posA, d.velA);
   return hipGetLastError();
 }
 
@@ -618,7 +650,7 @@ hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, 
 
 hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride) {
   k_nbr_export<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, x.d.posB, x.d.nbr,
-                                                         x.d.ncnt, x.p.max_neighbors, counts, ids,
+                                                         x.d.ncnt, k4_of(x), counts, ids,
                                                          stride);
   return hipGetLastError();
 }

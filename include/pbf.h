@@ -57,7 +57,9 @@ typedef struct pbf_params {
                               stop plane (no slide, strict crossing); the
                               other walls behave as the reference's mesh walls */
   float cell_scale;      /* grid cell edge = h * cell_scale, >= 1 */
-  int32_t max_neighbors; /* neighbour-list capacity per particle */
+  int32_t max_neighbors; /* neighbour-list capacity per particle, rounded up
+                            to a multiple of 4; more neighbours fail the step
+                            with PBF_ERR_OVERFLOW */
   float poly6_coef;      /* 315/(64 pi) as the reference literal (:140) */
   float spiky_coef;      /* 15/pi as the reference literal (:148) */
   float eps_d;           /* EPS_D (CGL/include/CGL/misc.h:11) */
