@@ -63,6 +63,9 @@ SIGNATURES = {
     "pbf_neighbor_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "pbf_grid_dims": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "pbf_step_debug": (C.c_int, [_P, C.c_float, C.c_int32]),
+    "pbf_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "pbf_step_async": (C.c_int, [_P, C.c_float, C.c_int32]),
+    "pbf_sync": (C.c_int, [_P]),
     "pbf_step_profiled": (C.c_int, [_P, C.c_float, C.c_int32, _F, C.POINTER(C.c_int32), C.c_int32]),
     "pbf_debug_read": (C.c_int, [_P, C.c_int32, _F]),
     "pbf_debug_derived": (C.c_int, [_P, _F, C.c_int32]),
@@ -81,6 +84,7 @@ ERRORS = {
 STAGE = {"predict": 1, "neighbors": 2, "lambda0": 3, "deltap0": 4, "solve": 5, "xsph": 6}
 BUF = {"pred": 1, "vel": 2, "vxsph": 3, "omega": 4, "pos": 5}
 PASSES = ["predict", "scan", "sort", "neighbors", "lambda", "deltap", "xsph", "vorticity"]
+SCHEDULE = {"flat": 0, "graph": 1, "persistent": 2}
 
 _lib = None
 

@@ -86,9 +86,17 @@ void free_device(Ctx* x) {
   dfree(d.cell); dfree(d.arrival); dfree(d.cellSorted); dfree(d.tmpId);
   dfree(d.cellCount); dfree(d.cellStart); dfree(d.blockSums);
   dfree(d.nbr); dfree(d.ncnt); dfree(d.flags);
-  dfree(d.grids); dfree(d.slotScene); dfree(d.stage);
+  dfree(d.grids); dfree(d.slotScene); dfree(d.runs); dfree(d.stage);
   x->alloc_n = 0;
   x->alloc_cells = 0;
+}
+
+// Drops the captured step graph (buffers, dt or schedule changed).
+void drop_graph(Ctx* x) {
+  if (x->graph_exec) (void)hipGraphExecDestroy(x->graph_exec);
+  if (x->graph) (void)hipGraphDestroy(x->graph);
+  x->graph_exec = nullptr;
+  x->graph = nullptr;
 }
 
 template <class T>
@@ -111,7 +119,8 @@ int ensure_device(Ctx* x) {
   DA(d.cell, np); DA(d.arrival, np); DA(d.cellSorted, np); DA(d.tmpId, np);
   DA(d.cellCount, C + 4); DA(d.cellStart, C + 4); DA(d.blockSums, nb);
   DA(d.nbr, np * K); DA(d.ncnt, np); DA(d.flags, 64);
-  DA(d.grids, (size_t)x->nscenes); DA(d.slotScene, np); DA(d.stage, 4 * np);
+  DA(d.grids, (size_t)x->nscenes); DA(d.slotScene, np); DA(d.runs, (size_t)x->nscenes);
+  DA(d.stage, 4 * np);
 #undef DA
   if (e != hipSuccess) {
     free_device(x);
@@ -137,6 +146,7 @@ int finish_create(Ctx* x, pbf_ctx** out) {
   x->ncell = (uint32_t)total;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&x->ncu, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     std::string m = std::string("HIP unavailable: ") + hipGetErrorString(e);
@@ -373,6 +383,7 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   if (!x) return PBF_OK;
   (void)hipSetDevice(x->device);
   if (x->stream) (void)hipStreamSynchronize(x->stream);
+  drop_graph(x);
   free_device(x);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
@@ -433,7 +444,22 @@ PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
                            hipMemcpyHostToDevice, x->stream));
   HIPCHK(x, hipMemcpyAsync(d.slotScene, slot_scene.data(), (size_t)n * sizeof(uint32_t),
                            hipMemcpyHostToDevice, x->stream));
+  // Scene runs for the persistent schedule, largest first (longest
+  // processing time first balances the workgroups).
+  std::vector<pbf::SceneRun> runs((size_t)x->nscenes);
+  for (int s = 0; s < x->nscenes; s++) {
+    const SceneGrid& g = x->grids[(size_t)s];
+    runs[(size_t)s] = {(uint32_t)s, (uint32_t)x->offsets[(size_t)s], (uint32_t)x->counts[(size_t)s],
+                       (uint32_t)g.dims[0] * (uint32_t)g.dims[1] * (uint32_t)g.dims[2]};
+  }
+  std::stable_sort(runs.begin(), runs.end(), [](const pbf::SceneRun& a, const pbf::SceneRun& b) {
+    return a.count > b.count;
+  });
+  HIPCHK(x, hipMemcpyAsync(d.runs, runs.data(), runs.size() * sizeof(pbf::SceneRun),
+                           hipMemcpyHostToDevice, x->stream));
+  HIPCHK(x, pbf::launch_fill_u32(d.flags, 64, 0u, x->stream));
   HIPCHK(x, hipStreamSynchronize(x->stream));
+  drop_graph(x);
   x->sim_time = 0.0;
   x->loaded = true;
   x->debug_dirty = false;
@@ -441,28 +467,103 @@ PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
   return PBF_OK;
 }
 
-PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
-  Ctx* x = C(c);
+namespace {
+
+// Waits for the stream, then reports (and clears) a neighbour-list overflow.
+int sync_check(Ctx* x) {
+  uint32_t over = 0;
+  int rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
+  if (rc != PBF_OK) return rc;
+  if (over == 0) return PBF_OK;
+  HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  return fail(x, PBF_ERR_OVERFLOW,
+              "a particle had " + std::to_string(over) + " neighbours; the list capacity is " +
+                  std::to_string(pbf::k4_of(*x) * pbf::kChunk) + " (max_neighbors " +
+                  std::to_string(x->p.max_neighbors) + ")");
+}
+
+// Enqueues nsteps steps with the context's schedule.
+int enqueue_steps(Ctx* x, float dt, int nsteps) {
+  if (nsteps == 0) return PBF_OK;
+  SimConst k;
+  pbf::derive_constants(x->p, dt, &k);
+  if (x->schedule == PBF_SCHED_PERSISTENT) {
+    const int per = pbf::persistent_max_blocks_per_cu(x->pblock);
+    const int grid = std::max(1, std::min(x->nscenes, std::max(1, x->ncu) * per));
+    HIPCHK(x, pbf::launch_persistent(*x, k, nsteps, x->pblock, grid));
+    x->has_nbrs = true;
+    x->dbg_vel = x->d.velA;
+  } else if (x->schedule == PBF_SCHED_GRAPH) {
+    if (!x->graph_exec || x->graph_dt != dt) {
+      drop_graph(x);
+      HIPCHK(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = run_step(x, k, 0);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(x->stream, &g);
+      if (rc != PBF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(x, PBF_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+      x->graph = g;
+      HIPCHK(x, hipGraphInstantiate(&x->graph_exec, g, nullptr, nullptr, 0));
+      x->graph_dt = dt;
+    }
+    for (int s = 0; s < nsteps; s++) HIPCHK(x, hipGraphLaunch(x->graph_exec, x->stream));
+  } else {
+    for (int s = 0; s < nsteps; s++) {
+      const int rc = run_step(x, k, 0);
+      if (rc != PBF_OK) return rc;
+    }
+  }
+  for (int s = 0; s < nsteps; s++) x->sim_time += (double)dt;
+  return PBF_OK;
+}
+
+int check_step_args(Ctx* x, float dt, int32_t nsteps) {
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
   if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
   if (!is_finite_f(dt) || !(dt > 0.0f) || nsteps < 0)
     return fail(x, PBF_ERR_INVALID, "dt must be finite and > 0, nsteps >= 0");
-  SimConst k;
-  pbf::derive_constants(x->p, dt, &k);
-  HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
-  for (int s = 0; s < nsteps; s++) {
-    rc = run_step(x, k, 0);
-    if (rc != PBF_OK) return rc;
-    x->sim_time += (double)dt;
-  }
-  uint32_t over = 0;
-  rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
+  return PBF_OK;
+}
+
+}  // namespace
+
+PBF_API int pbf_step_async(pbf_ctx* c, float dt, int32_t nsteps) {
+  Ctx* x = C(c);
+  int rc = check_step_args(x, dt, nsteps);
   if (rc != PBF_OK) return rc;
-  if (over > 0)
-    return fail(x, PBF_ERR_OVERFLOW,
-                "a particle had " + std::to_string(over) + " neighbours, max_neighbors is " +
-                    std::to_string(x->p.max_neighbors));
+  return enqueue_steps(x, dt, nsteps);
+}
+
+PBF_API int pbf_sync(pbf_ctx* c) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  return sync_check(x);
+}
+
+PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
+  Ctx* x = C(c);
+  int rc = check_step_args(x, dt, nsteps);
+  if (rc != PBF_OK) return rc;
+  if ((rc = enqueue_steps(x, dt, nsteps)) != PBF_OK) return rc;
+  return sync_check(x);
+}
+
+PBF_API int pbf_set_schedule(pbf_ctx* c, int32_t schedule, int32_t block) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (schedule < PBF_SCHED_FLAT || schedule > PBF_SCHED_PERSISTENT)
+    return fail(x, PBF_ERR_INVALID, "unknown schedule");
+  if (schedule == PBF_SCHED_PERSISTENT && block != 256 && block != 512 && block != 1024)
+    return fail(x, PBF_ERR_INVALID, "persistent block must be 256, 512 or 1024");
+  drop_graph(x);
+  x->schedule = schedule;
+  if (schedule == PBF_SCHED_PERSISTENT) x->pblock = block;
   return PBF_OK;
 }
 
@@ -500,14 +601,7 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
   }
   for (hipEvent_t e : pe.ev) (void)hipEventDestroy(e);
   if (rc != PBF_OK) return rc;
-  uint32_t over = 0;
-  rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
-  if (rc != PBF_OK) return rc;
-  if (over > 0)
-    return fail(x, PBF_ERR_OVERFLOW,
-                "a particle had " + std::to_string(over) + " neighbours, max_neighbors is " +
-                    std::to_string(x->p.max_neighbors));
-  return PBF_OK;
+  return sync_check(x);
 }
 
 PBF_API int pbf_step(pbf_ctx* c, int32_t nsteps) {
@@ -526,6 +620,7 @@ PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
   SimConst k;
   pbf::derive_constants(x->p, dt, &k);
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  drop_graph(x);
   x->debug_dirty = true;
   rc = run_step(x, k, stage);
   if (rc != PBF_OK) return rc;
@@ -565,6 +660,7 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
   HIPCHK(x, pbf::launch_densities(*x, k, x->d.stage));
   if ((rc = copy_out(x, x->d.stage, rho, (size_t)x->n * sizeof(float))) != PBF_OK) return rc;
   // The density pass re-sorted the committed state into posB/velB.
+  drop_graph(x);
   std::swap(x->d.posA, x->d.posB);
   std::swap(x->d.velA, x->d.velB);
   x->has_nbrs = false;

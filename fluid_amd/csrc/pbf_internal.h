@@ -13,6 +13,14 @@
 
 namespace pbf {
 
+// One scene of a multi-scene context as the persistent schedule sees it.
+struct SceneRun {
+  uint32_t scene;   // index into the grid table
+  uint32_t offset;  // first particle
+  uint32_t count;   // particles
+  uint32_t ncells;  // cells of its grid
+};
+
 // Device state of one context.  All particle arrays are float4 and are kept
 // in cell-sorted order between steps; posA.w carries the particle's load
 // index (its id) as integer bits.
@@ -37,6 +45,7 @@ struct DevState {
   uint32_t* flags = nullptr;      // [0] max neighbour count seen beyond K
   SceneGrid* grids = nullptr;     // per-scene grids (multi-scene)
   uint32_t* slotScene = nullptr;  // scene of each sorted slot (multi-scene)
+  SceneRun* runs = nullptr;       // scenes, largest first (persistent schedule)
   float* stage = nullptr;         // read-back staging, n * 4 floats
 };
 
@@ -61,6 +70,12 @@ struct Ctx {
   bool has_nbrs = false;
   const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop
   const float4* dbg_vel = nullptr;   // velocity buffer current at the last debug stop
+  int schedule = PBF_SCHED_FLAT;     // PBF_SCHED_*
+  int pblock = 256;                  // workgroup size of the persistent schedule
+  int ncu = 0;                       // compute units of the device
+  hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)
+  hipGraphExec_t graph_exec = nullptr;
+  float graph_dt = 0.0f;
   std::string err;
 };
 
@@ -89,6 +104,8 @@ hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int fi
 hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, uint32_t* dst);
 hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride);
 hipError_t launch_densities(Ctx& x, const SimConst& c, float* out);
+hipError_t launch_persistent(Ctx& x, const SimConst& c, int nsteps, int block, int grid);
+int persistent_max_blocks_per_cu(int block);
 hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx);
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
                            hipStream_t s);

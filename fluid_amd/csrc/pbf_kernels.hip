@@ -1,17 +1,23 @@
 // pbf_kernels.hip -- gfx950 kernels of the PBF step and their launchers.
 //
 // One step (the reference's Particles::timeStep, src/particles.cpp:250-297):
-//   k_predict        gravity, x* = x, clamp_response, cell hash, histogram
-//   k_scan_*         wave64 exclusive scan of the cell histogram
-//   k_scatter_ids    counting-sort scatter of particle ids by cell
-//   k_rank_reorder   deterministic in-cell rank by id, gather into sorted order
-//   k_nbr_build      27-cell search -> tiled ELL neighbour lists (fixed per step)
-//   k_lambda         density / lambda pass            (x S)
-//   k_deltap         delta-p + clamp pass              (x S; last one also v)
-//   k_xsph           XSPH viscosity, vorticity, density
-//   k_vort_commit    vorticity confinement, x <- x*
+//   predict        gravity, x* = x, clamp_response, cell hash, histogram
+//   scan           wave64 exclusive scan of the cell histogram
+//   scatter/rank   counting sort by cell, deterministic in-cell rank by id,
+//                  gather into sorted order
+//   nbr_build      27-cell search -> chunked ELL neighbour lists (fixed per step)
+//   lambda         density / lambda pass            (x S)
+//   deltap         delta-p + clamp pass              (x S; last one also v)
+//   xsph           XSPH viscosity, vorticity, density
+//   vort_commit    vorticity confinement, x <- x*
 // Particles stay in the order (cell, id), so the lists of a particle are in
 // ascending sorted index, i.e. the same order a CPU visits the 27 cells in.
+//
+// Every pass is a per-particle device function.  The flat schedule runs one
+// launch per pass over all particles of all scenes; the persistent schedule
+// runs whole scenes inside one workgroup (all passes, all steps, workgroup
+// barriers between passes) in a single launch.  Both call the same
+// functions, so both give the same bits.
 #include <hip/hip_runtime.h>
 
 #include "pbf_internal.h"
@@ -30,6 +36,29 @@ constexpr int kBlock = 256;
 #endif
 constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
 constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers two float4 per neighbour
+
+// Device pointers and sizes of one context, passed by value to every kernel.
+struct Dev {
+  float4 *posA, *posB, *velA, *velB, *predP, *predQ, *vort, *vxsph;
+  uint32_t *cell, *arrival, *cellSorted, *tmpId, *cellCount, *cellStart, *nbr, *ncnt, *flags;
+  const SceneGrid* grids;
+  const uint32_t* slotScene;
+  uint32_t n;
+  int K4;
+};
+
+Dev dev_of(const Ctx& x) {
+  const DevState& s = x.d;
+  Dev d;
+  d.posA = s.posA; d.posB = s.posB; d.velA = s.velA; d.velB = s.velB;
+  d.predP = s.predP; d.predQ = s.predQ; d.vort = s.vort; d.vxsph = s.vxsph;
+  d.cell = s.cell; d.arrival = s.arrival; d.cellSorted = s.cellSorted; d.tmpId = s.tmpId;
+  d.cellCount = s.cellCount; d.cellStart = s.cellStart; d.nbr = s.nbr; d.ncnt = s.ncnt;
+  d.flags = s.flags; d.grids = s.grids; d.slotScene = s.slotScene;
+  d.n = (uint32_t)x.n;
+  d.K4 = k4_of(x);
+  return d;
+}
 
 __device__ __forceinline__ uint32_t id_of(const float4& p) { return __float_as_uint(p.w); }
 
@@ -127,15 +156,61 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, s
   for (; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
 }
 
-// ---------------------------------------------------------- grid build --
-template <bool MULTI>
-__device__ __forceinline__ SceneGrid grid_of(uint32_t i, const SceneGrid& g0,
-                                             const SceneGrid* __restrict__ grids,
-                                             const uint32_t* __restrict__ slotScene) {
-  if (MULTI) return grids[slotScene[i]];
-  return g0;
+// Agent-scope loads and stores of the cell histogram inside the persistent
+// kernel: the histogram is built with device-scope atomics, so it is read and
+// cleared with atomic accesses rather than through the per-CU cache.
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Barrier between two passes of the persistent kernel: every write of the
+// pass is released to L2 before the barrier, and the per-CU cache is
+// invalidated after it, so the next pass reads what the other waves wrote.
+__device__ __forceinline__ void phase_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Exclusive scan of in[0..n) into out[0..n) inside one workgroup of BLOCK
+// threads, every value offset by base; out[n] = base + total.  All threads
+// of the block must call it.
+template <int BLOCK>
+__device__ void block_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t n,
+                           uint32_t base, uint32_t* wsum /* BLOCK/64 + 1 shared */) {
+  constexpr int W = BLOCK / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t carry = base;
+  for (uint32_t b = 0; b < n; b += BLOCK * kScanItems) {
+    const uint32_t e0 = b + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) v[k] = (e0 + k < n) ? load_agent(in + e0 + k) : 0u;
+    const uint32_t t = (v[0] + v[1]) + (v[2] + v[3]);
+    const uint32_t incl = wave_incl_scan(t);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0, total = 0;
+    for (int w = 0; w < W; w++) {
+      if (w < wid) woff += wsum[w];
+      total += wsum[w];
+    }
+    uint32_t e = carry + woff + incl - t;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      if (e0 + k < n) out[e0 + k] = e;
+      e += v[k];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+// ----------------------------------------------------- per-particle passes --
 __device__ __forceinline__ uint32_t cell_lin(const SceneGrid& g, float3 p, int* cx = nullptr) {
   int ix = axis_cell(p.x, g.lo[0], g.inv_cell, g.dims[0]);
   int iy = axis_cell(p.y, g.lo[1], g.inv_cell, g.dims[1]);
@@ -151,63 +226,45 @@ __device__ __forceinline__ uint32_t cell_lin(const SceneGrid& g, float3 p, int* 
 
 // applyForceVelocity (src/particles.cpp:175-183) + cell hash + histogram.
 // With MOVE=false x* = x (used by the load-time density pass).
-template <bool MULTI, bool MOVE>
-__global__ __launch_bounds__(kBlock) void k_predict(
-    uint32_t n, const float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ pred,
-    uint32_t* __restrict__ cell, uint32_t* __restrict__ arrival, uint32_t* __restrict__ cellCount,
-    SimConst c, SceneGrid g0, const SceneGrid* __restrict__ grids,
-    const uint32_t* __restrict__ slotScene) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const SceneGrid g = grid_of<MULTI>(i, g0, grids, slotScene);
-  const float4 x = pos[i];
+template <bool MOVE>
+__device__ __forceinline__ void predict_one(uint32_t i, const Dev& d, const SimConst& c,
+                                            const SceneGrid& g) {
+  const float4 x = d.posA[i];
   float3 p = make_float3(x.x, x.y, x.z);
   if (MOVE) {
-    float4 v = vel[i];
+    float4 v = d.velA[i];
     v.y = v.y - c.gdt;
     p = clamp_response(g, c, p, make_float3(v.x, v.y, v.z));
-    vel[i] = v;
+    d.velA[i] = v;
   }
-  pred[i] = make_float4(p.x, p.y, p.z, 0.0f);
+  d.predP[i] = make_float4(p.x, p.y, p.z, 0.0f);
   const uint32_t cl = cell_lin(g, p);
-  cell[i] = cl;
-  arrival[i] = atomicAdd(&cellCount[cl], 1u);
+  d.cell[i] = cl;
+  d.arrival[i] = atomicAdd(&d.cellCount[cl], 1u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter_ids(uint32_t n, const uint32_t* __restrict__ cell,
-                                                        const uint32_t* __restrict__ arrival,
-                                                        const uint32_t* __restrict__ cellStart,
-                                                        const float4* __restrict__ pos,
-                                                        uint32_t* __restrict__ tmpId) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t slot = cellStart[cell[i]] + arrival[i];
-  if (slot < n) tmpId[slot] = id_of(pos[i]);
+__device__ __forceinline__ void scatter_one(uint32_t i, const Dev& d) {
+  const uint32_t slot = d.cellStart[d.cell[i]] + d.arrival[i];
+  if (slot < d.n) d.tmpId[slot] = id_of(d.posA[i]);
 }
 
 // Rank of a particle inside its cell = number of cell members with a smaller
 // id; the gather target is cellStart + rank, which makes the order (cell, id)
 // independent of atomic arrival order.
-__global__ __launch_bounds__(kBlock) void k_rank_reorder(
-    uint32_t n, const uint32_t* __restrict__ cell, const uint32_t* __restrict__ cellStart,
-    const uint32_t* __restrict__ tmpId, const float4* __restrict__ posA,
-    const float4* __restrict__ velA, const float4* __restrict__ predP, float4* __restrict__ posB,
-    float4* __restrict__ velB, float4* __restrict__ predQ, uint32_t* __restrict__ cellSorted) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t c = cell[i];
-  const uint32_t s = cellStart[c];
-  const uint32_t e = min(cellStart[c + 1], n);
-  const float4 x = posA[i];
+__device__ __forceinline__ void rank_one(uint32_t i, const Dev& d) {
+  const uint32_t c = d.cell[i];
+  const uint32_t s = d.cellStart[c];
+  const uint32_t e = min(d.cellStart[c + 1], d.n);
+  const float4 x = d.posA[i];
   const uint32_t id = id_of(x);
   uint32_t rank = 0;
-  for (uint32_t t = s; t < e; t++) rank += (tmpId[t] < id) ? 1u : 0u;
+  for (uint32_t t = s; t < e; t++) rank += (d.tmpId[t] < id) ? 1u : 0u;
   const uint32_t dst = s + rank;
-  if (dst >= n) return;
-  posB[dst] = x;
-  velB[dst] = velA[i];
-  predQ[dst] = predP[i];
-  cellSorted[dst] = c;
+  if (dst >= d.n) return;
+  d.posB[dst] = x;
+  d.velB[dst] = d.velA[i];
+  d.predQ[dst] = d.predP[i];
+  d.cellSorted[dst] = c;
 }
 
 // Neighbour sets of src/particles.cpp:258-265 (|x*_i - x*_j|^2 <= H2, j != i),
@@ -215,27 +272,20 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(
 // -1..1 are contiguous in sorted order, so the search walks 9 index ranges in
 // ascending order.  Found indices collect in a 4-entry register chunk that
 // is stored whole (one 16-byte store) when full.
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_nbr_build(
-    uint32_t n, const float4* __restrict__ pred, const uint32_t* __restrict__ cellStart,
-    uint32_t* __restrict__ nbr, uint32_t* __restrict__ ncnt, uint32_t* __restrict__ flags, int K4,
-    float h2, SceneGrid g0, const SceneGrid* __restrict__ grids,
-    const uint32_t* __restrict__ slotScene) {
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const SceneGrid g = grid_of<MULTI>(i, g0, grids, slotScene);
-  const float4 xi = pred[i];
+__device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, const SceneGrid& g) {
+  const float4 xi = d.predQ[i];
   int cc[3];
   cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
   const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
   const int x1 = cc[0] + 1 < g.dims[0] ? cc[0] + 1 : g.dims[0] - 1;
+  const float4* __restrict__ pred = d.predQ;
   uint32_t cnt = 0;
   uint32_t buf[kChunk];
 #pragma unroll
   for (int u = 0; u < kChunk; u++) buf[u] = 0u;
   auto flush = [&](int chunk) {
-    if (chunk < K4)
-      *reinterpret_cast<uint4*>(nbr + nbr_chunk(i, chunk, K4)) =
+    if (chunk < d.K4)
+      *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
           make_uint4(buf[0], buf[1], buf[2], buf[3]);
   };
   auto test = [&](uint32_t j, const float4& xj) {
@@ -257,41 +307,37 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(
       if (y < 0 || y >= g.dims[1]) continue;
       const uint32_t row =
           g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
-      const uint32_t s = cellStart[row + (uint32_t)x0];
-      const uint32_t e = min(cellStart[row + (uint32_t)x1 + 1u], n);
+      const uint32_t s = d.cellStart[row + (uint32_t)x0];
+      const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
       uint32_t j = s;
       for (; j + 4 <= e; j += 4) {
-        const float4 a = pred[j], b = pred[j + 1], c4 = pred[j + 2], d = pred[j + 3];
+        const float4 a = pred[j], b = pred[j + 1], c4 = pred[j + 2], q = pred[j + 3];
         test(j, a);
         test(j + 1, b);
         test(j + 2, c4);
-        test(j + 3, d);
+        test(j + 3, q);
       }
       for (; j < e; j++) test(j, pred[j]);
     }
   }
   if (cnt & 3u) flush((int)(cnt >> 2));
-  const uint32_t cap = (uint32_t)K4 * kChunk;
+  const uint32_t cap = (uint32_t)d.K4 * kChunk;
   if (cnt > cap) {
-    atomicMax(&flags[0], cnt);
+    atomicMax(&d.flags[0], cnt);
     cnt = cap;
   }
-  ncnt[i] = cnt;
+  d.ncnt[i] = cnt;
 }
 
-// ------------------------------------------------------------ solver ----
 // newtonStepCalculateLambda (src/particles.cpp:185-204).  out = (x*, lambda).
-__global__ __launch_bounds__(kBlock) void k_lambda(uint32_t n, const float4* __restrict__ in,
-                                                   const uint32_t* __restrict__ nbr,
-                                                   const uint32_t* __restrict__ ncnt, int K4,
-                                                   SimConst c, float4* __restrict__ out) {
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimConst& c,
+                                           const float4* __restrict__ in,
+                                           float4* __restrict__ out) {
   const float4 xi = in[i];
-  const int cnt = (int)ncnt[i];
+  const int cnt = (int)d.ncnt[i];
   float dens = 0.0f, denom = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
   for_each_neighbor<kUnroll>(
-      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -315,19 +361,15 @@ __global__ __launch_bounds__(kBlock) void k_lambda(uint32_t n, const float4* __r
 // newtonStepUpdatePosition (src/particles.cpp:206-213) with s_corr (:195)
 // and clamp (:51-84).  in = (x*, lambda), out = (x*', 0).  The last
 // iteration also forms v = (x* - x) / dt (updateVelocity, :215-217).
-template <bool MULTI, bool LAST>
-__global__ __launch_bounds__(kBlock) void k_deltap(
-    uint32_t n, const float4* __restrict__ in, const uint32_t* __restrict__ nbr,
-    const uint32_t* __restrict__ ncnt, int K4, SimConst c, SceneGrid g0,
-    const SceneGrid* __restrict__ grids, const uint32_t* __restrict__ slotScene,
-    float4* __restrict__ out, const float4* __restrict__ pos, float4* __restrict__ vel) {
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+template <bool LAST>
+__device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimConst& c,
+                                           const SceneGrid& g, const float4* __restrict__ in,
+                                           float4* __restrict__ out) {
   const float4 xi = in[i];
-  const int cnt = (int)ncnt[i];
+  const int cnt = (int)d.ncnt[i];
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
   for_each_neighbor<kUnroll>(
-      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -342,40 +384,34 @@ __global__ __launch_bounds__(kBlock) void k_deltap(
   dx = dx * c.inv_rho0;
   dy = dy * c.inv_rho0;
   dz = dz * c.inv_rho0;
-  const SceneGrid g = grid_of<MULTI>(i, g0, grids, slotScene);
   const float3 p = clamp_dp(g, c, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
   out[i] = make_float4(p.x, p.y, p.z, 0.0f);
   if (LAST) {
-    const float4 x = pos[i];
-    vel[i] = make_float4(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y), c.inv_dt * (p.z - x.z),
-                         0.0f);
+    const float4 x = d.posB[i];
+    d.velB[i] = make_float4(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y),
+                            c.inv_dt * (p.z - x.z), 0.0f);
   }
 }
 
 // calculateVorticityApplyXSPHViscosity (src/particles.cpp:219-234), Jacobi:
 // every neighbour velocity is the pre-XSPH v.  out = (x*, |omega|),
-// vx = (v + C*visc, density), vort = (omega, 0).
-__global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, const float4* __restrict__ in,
-                                                 const float4* __restrict__ vel,
-                                                 const uint32_t* __restrict__ nbr,
-                                                 const uint32_t* __restrict__ ncnt, int K4,
-                                                 SimConst c, float4* __restrict__ out,
-                                                 float4* __restrict__ vx,
-                                                 float4* __restrict__ vort) {
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+// vxsph = (v + C*visc, density), vort = (omega, 0).
+__device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimConst& c,
+                                         const float4* __restrict__ in,
+                                         float4* __restrict__ out) {
+  const float4* __restrict__ vel = d.velB;
   const float4 xi = in[i];
   const float4 vi = vel[i];
-  const int cnt = (int)ncnt[i];
+  const int cnt = (int)d.ncnt[i];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, dens = 0.0f;
   struct XV {
     float4 x, v;
   };
   for_each_neighbor<kUnrollXsph>(
-      i, cnt, K4, nbr,[&](uint32_t j) { return XV{in[j], vel[j]}; },
-      [&](uint32_t, const XV& d) {
-        const float u0 = d.v.x - vi.x, u1 = d.v.y - vi.y, u2 = d.v.z - vi.z;
-        const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{in[j], vel[j]}; },
+      [&](uint32_t, const XV& q) {
+        const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
+        const float rx = xi.x - q.x.x, ry = xi.y - q.x.y, rz = xi.z - q.x.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const float3 gr = spiky_sel(c, rx, ry, rz, r2);
         ox = ox + (u1 * gr.z - u2 * gr.y);
@@ -389,24 +425,19 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, const float4* __res
       });
   const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
   out[i] = make_float4(xi.x, xi.y, xi.z, wn);
-  vx[i] = make_float4(vi.x + c.xc * sx, vi.y + c.xc * sy, vi.z + c.xc * sz, dens);
-  vort[i] = make_float4(ox, oy, oz, 0.0f);
+  d.vxsph[i] = make_float4(vi.x + c.xc * sx, vi.y + c.xc * sy, vi.z + c.xc * sz, dens);
+  d.vort[i] = make_float4(ox, oy, oz, 0.0f);
 }
 
 // applyVorticity + updatePosition (src/particles.cpp:236-248).
 // in = (x*, |omega|).  Commits posA = (x*, id), velA = (v, density).
-__global__ __launch_bounds__(kBlock) void k_vort_commit(
-    uint32_t n, const float4* __restrict__ in, const float4* __restrict__ vort,
-    const float4* __restrict__ vx, const float4* __restrict__ posB,
-    const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ ncnt, int K4, SimConst c,
-    float4* __restrict__ posA, float4* __restrict__ velA) {
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimConst& c,
+                                         const float4* __restrict__ in) {
   const float4 xi = in[i];
-  const int cnt = (int)ncnt[i];
+  const int cnt = (int)d.ncnt[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   for_each_neighbor<kUnroll>(
-      i, cnt, K4, nbr,[&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -415,12 +446,12 @@ __global__ __launch_bounds__(kBlock) void k_vort_commit(
         ey = ey + xj.w * gr.y;
         ez = ez + xj.w * gr.z;
       });
-  float4 v = vx[i];
+  float4 v = d.vxsph[i];
   const float e2 = ex * ex + ey * ey + ez * ez;
   if (sqrtf(e2) > c.eps_d) {
     const float rn = 1.0f / sqrtf(e2);
     const float u0 = rn * ex, u1 = rn * ey, u2 = rn * ez;
-    const float4 w = vort[i];
+    const float4 w = d.vort[i];
     const float c0 = u1 * w.z - u2 * w.y;
     const float c1 = u2 * w.x - u0 * w.z;
     const float c2 = u0 * w.y - u1 * w.x;
@@ -428,21 +459,15 @@ __global__ __launch_bounds__(kBlock) void k_vort_commit(
     v.y = v.y + c.dtv * c1;
     v.z = v.z + c.dtv * c2;
   }
-  posA[i] = make_float4(xi.x, xi.y, xi.z, posB[i].w);
-  velA[i] = v;
+  d.posA[i] = make_float4(xi.x, xi.y, xi.z, d.posB[i].w);
+  d.velA[i] = v;
 }
 
 // Load-time density (src/particles.cpp:440-444 -> :158-163): every particle
 // within the 27 cells, self included, r = x_j - x_i.  Written by id.
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_densities(
-    uint32_t n, const float4* __restrict__ pred, const float4* __restrict__ posB,
-    const uint32_t* __restrict__ cellStart, SimConst c, SceneGrid g0,
-    const SceneGrid* __restrict__ grids, const uint32_t* __restrict__ slotScene,
-    float* __restrict__ out) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const SceneGrid g = grid_of<MULTI>(i, g0, grids, slotScene);
+__device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimConst& c,
+                                            const SceneGrid& g, float* __restrict__ out) {
+  const float4* __restrict__ pred = d.predQ;
   const float4 xi = pred[i];
   int cc[3];
   cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
@@ -457,8 +482,8 @@ __global__ __launch_bounds__(kBlock) void k_densities(
       if (y < 0 || y >= g.dims[1]) continue;
       const uint32_t row =
           g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
-      const uint32_t s = cellStart[row + (uint32_t)x0];
-      const uint32_t e = min(cellStart[row + (uint32_t)x1 + 1u], n);
+      const uint32_t s = d.cellStart[row + (uint32_t)x0];
+      const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
       for (uint32_t j = s; j < e; j++) {
         const float4 xj = pred[j];
         const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
@@ -466,7 +491,128 @@ __global__ __launch_bounds__(kBlock) void k_densities(
       }
     }
   }
-  out[id_of(posB[i])] = dens;
+  out[id_of(d.posB[i])] = dens;
+}
+
+// ------------------------------------------------------- flat schedule --
+template <bool MULTI>
+__device__ __forceinline__ SceneGrid grid_of(uint32_t i, const SceneGrid& g0, const Dev& d) {
+  if (MULTI) return d.grids[d.slotScene[i]];
+  return g0;
+}
+
+__device__ __forceinline__ uint32_t flat_index() { return blockIdx.x * kBlock + threadIdx.x; }
+// Gather-heavy passes take the XCD-aware block order.
+__device__ __forceinline__ uint32_t gather_index() {
+  return xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+}
+
+template <bool MULTI, bool MOVE>
+__global__ __launch_bounds__(kBlock) void k_predict(Dev d, SimConst c, SceneGrid g0) {
+  const uint32_t i = flat_index();
+  if (i < d.n) predict_one<MOVE>(i, d, c, grid_of<MULTI>(i, g0, d));
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(Dev d) {
+  const uint32_t i = flat_index();
+  if (i < d.n) scatter_one(i, d);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank(Dev d) {
+  const uint32_t i = flat_index();
+  if (i < d.n) rank_one(i, d);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_nbr_build(Dev d, float h2, SceneGrid g0) {
+  const uint32_t i = gather_index();
+  if (i < d.n) nbr_one(i, d, h2, grid_of<MULTI>(i, g0, d));
+}
+
+__global__ __launch_bounds__(kBlock) void k_lambda(Dev d, SimConst c, const float4* in,
+                                                   float4* out) {
+  const uint32_t i = gather_index();
+  if (i < d.n) lambda_one(i, d, c, in, out);
+}
+
+template <bool MULTI, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_deltap(Dev d, SimConst c, SceneGrid g0,
+                                                   const float4* in, float4* out) {
+  const uint32_t i = gather_index();
+  if (i < d.n) deltap_one<LAST>(i, d, c, grid_of<MULTI>(i, g0, d), in, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_xsph(Dev d, SimConst c, const float4* in,
+                                                 float4* out) {
+  const uint32_t i = gather_index();
+  if (i < d.n) xsph_one(i, d, c, in, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vort_commit(Dev d, SimConst c, const float4* in) {
+  const uint32_t i = gather_index();
+  if (i < d.n) vort_one(i, d, c, in);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_densities(Dev d, SimConst c, SceneGrid g0,
+                                                      float* out) {
+  const uint32_t i = flat_index();
+  if (i < d.n) density_one(i, d, c, grid_of<MULTI>(i, g0, d), out);
+}
+
+// ------------------------------------------------- persistent schedule --
+// One launch advances every scene by nsteps steps.  Workgroups take scenes
+// from a queue in the host's order (largest first); a workgroup runs all
+// passes of all steps of its scene, with workgroup barriers between passes.
+// A scene's particles and cells are contiguous ranges of the global arrays,
+// so every pass writes exactly what the flat schedule writes.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const SceneRun* runs,
+                                                      uint32_t nruns, uint32_t* queue,
+                                                      int iterations, int nsteps) {
+  __shared__ uint32_t wsum[BLOCK / 64 + 1];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    __syncthreads();
+    if (item >= nruns) break;
+    const SceneRun r = runs[item];
+    const SceneGrid g = d.grids[r.scene];
+    const uint32_t b = r.offset, e = r.offset + r.count;
+    const uint32_t c0 = g.cell_base, nc = r.ncells;
+    for (int step = 0; step < nsteps; step++) {
+      for (uint32_t k = threadIdx.x; k < nc; k += BLOCK) store_agent(d.cellCount + c0 + k, 0u);
+      phase_barrier();
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) predict_one<true>(i, d, c, g);
+      phase_barrier();
+      block_scan<BLOCK>(d.cellCount + c0, d.cellStart + c0, nc, b, wsum);
+      phase_barrier();
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) scatter_one(i, d);
+      phase_barrier();
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) rank_one(i, d);
+      phase_barrier();
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) nbr_one(i, d, c.h2, g);
+      phase_barrier();
+      for (int it = 0; it < iterations; it++) {
+        for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) lambda_one(i, d, c, d.predQ, d.predP);
+        phase_barrier();
+        if (it == iterations - 1) {
+          for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK)
+            deltap_one<true>(i, d, c, g, d.predP, d.predQ);
+        } else {
+          for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK)
+            deltap_one<false>(i, d, c, g, d.predP, d.predQ);
+        }
+        phase_barrier();
+      }
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) xsph_one(i, d, c, d.predQ, d.predP);
+      phase_barrier();
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) vort_one(i, d, c, d.predP);
+      phase_barrier();
+    }
+  }
 }
 
 // ------------------------------------------------------------ read-back --
@@ -490,21 +636,19 @@ __global__ __launch_bounds__(kBlock) void k_unpermute_u32(uint32_t n,
   dst[id_of(ids[i])] = src[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_nbr_export(uint32_t n, const float4* __restrict__ ids,
-                                                       const uint32_t* __restrict__ nbr,
-                                                       const uint32_t* __restrict__ ncnt, int K4,
-                                                       uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(kBlock) void k_nbr_export(Dev d, uint32_t* __restrict__ counts,
                                                        uint32_t* __restrict__ out, int stride) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t id = id_of(ids[i]);
-  const int cnt = (int)ncnt[i];
+  if (i >= d.n) return;
+  const uint32_t id = id_of(d.posB[i]);
+  const int cnt = (int)d.ncnt[i];
   counts[id] = (uint32_t)cnt;
   const int m = cnt < stride ? cnt : stride;
-  for (int k = 0; k < m; k++) out[(size_t)id * stride + k] = id_of(ids[nbr[nbr_slot(i, k, K4)]]);
+  for (int k = 0; k < m; k++)
+    out[(size_t)id * stride + k] = id_of(d.posB[d.nbr[nbr_slot(i, k, d.K4)]]);
 }
 
-// Sum and maximum of the neighbour counts (out[0..1] = sum lo/hi, out[2] = max).
+// Sum and maximum of the neighbour counts (64-bit sum, maximum).
 __global__ __launch_bounds__(kBlock) void k_nbr_stats(uint32_t n, const uint32_t* __restrict__ ncnt,
                                                       unsigned long long* __restrict__ sum,
                                                       uint32_t* __restrict__ mx) {
@@ -545,18 +689,13 @@ hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32
 static bool multi(const Ctx& x) { return x.nscenes > 1; }
 
 hipError_t launch_predict(Ctx& x, const SimConst& c) {
-  DevState& d = x.d;
-  hipError_t e = launch_fill_u32(d.cellCount, x.ncell, 0u, x.stream);
+  hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
   if (e != hipSuccess) return e;
-  const unsigned nb = blocks_for(x.n);
+  const Dev d = dev_of(x);
   if (multi(x))
-    k_predict<true, true><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.posA, d.velA, d.predP,
-                                                       d.cell, d.arrival, d.cellCount, c,
-                                                       x.grids[0], d.grids, d.slotScene);
+    k_predict<true, true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0]);
   else
-    k_predict<false, true><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.posA, d.velA, d.predP,
-                                                        d.cell, d.arrival, d.cellCount, c,
-                                                        x.grids[0], d.grids, d.slotScene);
+    k_predict<false, true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0]);
   return hipGetLastError();
 }
 
@@ -565,75 +704,91 @@ hipError_t launch_scan(Ctx& x) {
 }
 
 hipError_t launch_scatter_rank(Ctx& x) {
-  DevState& d = x.d;
-  const unsigned nb = blocks_for(x.n);
-  k_scatter_ids<<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.cell, d.arrival, d.cellStart,
-                                             d.posA, d.tmpId);
-  k_rank_reorder<<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.cell, d.cellStart, d.tmpId,
-                                              d.posA, d.velA, d.predP, d.posB, d.velB, d.predQ,
-                                              d.cellSorted);
+  const Dev d = dev_of(x);
+  k_scatter<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d);
+  k_rank<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d);
   return hipGetLastError();
 }
 
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
-  DevState& d = x.d;
-  const unsigned nb = blocks_for(x.n);
+  const Dev d = dev_of(x);
   if (multi(x))
-    k_nbr_build<true><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.cellStart, d.nbr,
-                                                   d.ncnt, d.flags, k4_of(x), c.h2,
-                                                   x.grids[0], d.grids, d.slotScene);
+    k_nbr_build<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
   else
-    k_nbr_build<false><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.cellStart, d.nbr,
-                                                    d.ncnt, d.flags, k4_of(x), c.h2,
-                                                    x.grids[0], d.grids, d.slotScene);
+    k_nbr_build<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
   return hipGetLastError();
 }
 
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out) {
-  k_lambda<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, x.d.nbr, x.d.ncnt,
-                                                     k4_of(x), c, out);
+  k_lambda<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
   return hipGetLastError();
 }
 
 hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last) {
-  DevState& d = x.d;
+  const Dev d = dev_of(x);
   const unsigned nb = blocks_for(x.n);
-  const uint32_t n = (uint32_t)x.n;
-  const int K = k4_of(x);
   if (multi(x)) {
     if (last)
-      k_deltap<true, true><<<nb, kBlock, 0, x.stream>>>(n, in, d.nbr, d.ncnt, K, c, x.grids[0],
-                                                        d.grids, d.slotScene, out, d.posB, d.velB);
+      k_deltap<true, true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
     else
-      k_deltap<true, false><<<nb, kBlock, 0, x.stream>>>(n, in, d.nbr, d.ncnt, K, c, x.grids[0],
-                                                         d.grids, d.slotScene, out, d.posB,
-                                                         d.velB);
+      k_deltap<true, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
   } else {
     if (last)
-      k_deltap<false, true><<<nb, kBlock, 0, x.stream>>>(n, in, d.nbr, d.ncnt, K, c, x.grids[0],
-                                                         d.grids, d.slotScene, out, d.posB,
-                                                         d.velB);
+      k_deltap<false, true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
     else
-      k_deltap<false, false><<<nb, kBlock, 0, x.stream>>>(n, in, d.nbr, d.ncnt, K, c, x.grids[0],
-                                                          d.grids, d.slotScene, out, d.posB,
-                                                          d.velB);
+      k_deltap<false, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out) {
-  DevState& d = x.d;
-  k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, d.velB, d.nbr, d.ncnt,
-                                                   k4_of(x), c, out, d.vxsph, d.vort);
+  k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
   return hipGetLastError();
 }
 
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
-  DevState& d = x.d;
-  k_vort_commit<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, in, d.vort, d.vxsph,
-                                                          d.posB, d.nbr, d.ncnt,
-                                                          k4_of(x), c, d.posA, d.velA);
+  k_vort_commit<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in);
   return hipGetLastError();
+}
+
+hipError_t launch_persistent(Ctx& x, const SimConst& c, int nsteps, int block, int grid) {
+  const Dev d = dev_of(x);
+  uint32_t* queue = x.d.flags + 8;
+  hipError_t e = launch_fill_u32(queue, 1, 0u, x.stream);
+  if (e != hipSuccess) return e;
+  const uint32_t nr = (uint32_t)x.nscenes;
+  switch (block) {
+    case 1024:
+      k_persistent<1024><<<grid, 1024, 0, x.stream>>>(d, c, x.d.runs, nr, queue, x.p.iterations,
+                                                      nsteps);
+      break;
+    case 512:
+      k_persistent<512><<<grid, 512, 0, x.stream>>>(d, c, x.d.runs, nr, queue, x.p.iterations,
+                                                    nsteps);
+      break;
+    default:
+      k_persistent<256><<<grid, 256, 0, x.stream>>>(d, c, x.d.runs, nr, queue, x.p.iterations,
+                                                    nsteps);
+      break;
+  }
+  return hipGetLastError();
+}
+
+int persistent_max_blocks_per_cu(int block) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (block) {
+    case 1024:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_persistent<1024>, 1024, 0);
+      break;
+    case 512:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_persistent<512>, 512, 0);
+      break;
+    default:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_persistent<256>, 256, 0);
+      break;
+  }
+  return (e == hipSuccess && nb > 0) ? nb : 1;
 }
 
 hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int first, int count,
@@ -649,9 +804,7 @@ hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, 
 }
 
 hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride) {
-  k_nbr_export<<<blocks_for(x.n), kBlock, 0, x.stream>>>((uint32_t)x.n, x.d.posB, x.d.nbr,
-                                                         x.d.ncnt, k4_of(x), counts, ids,
-                                                         stride);
+  k_nbr_export<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), counts, ids, stride);
   return hipGetLastError();
 }
 
@@ -665,27 +818,21 @@ hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx) {
 // Load-time density: hash the committed positions (no move), sort, then sum
 // poly6 over the 27 cells.  Leaves the state re-sorted (posB/velB hold it).
 hipError_t launch_densities(Ctx& x, const SimConst& c, float* out) {
-  DevState& d = x.d;
-  hipError_t e = launch_fill_u32(d.cellCount, x.ncell, 0u, x.stream);
+  hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
   if (e != hipSuccess) return e;
   const unsigned nb = blocks_for(x.n);
+  Dev d = dev_of(x);
   if (multi(x))
-    k_predict<true, false><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.posA, d.velA, d.predP,
-                                                        d.cell, d.arrival, d.cellCount, c,
-                                                        x.grids[0], d.grids, d.slotScene);
+    k_predict<true, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0]);
   else
-    k_predict<false, false><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.posA, d.velA, d.predP,
-                                                         d.cell, d.arrival, d.cellCount, c,
-                                                         x.grids[0], d.grids, d.slotScene);
+    k_predict<false, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0]);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(x)) != hipSuccess) return e;
   if ((e = launch_scatter_rank(x)) != hipSuccess) return e;
   if (multi(x))
-    k_densities<true><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.posB, d.cellStart, c,
-                                                   x.grids[0], d.grids, d.slotScene, out);
+    k_densities<true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
   else
-    k_densities<false><<<nb, kBlock, 0, x.stream>>>((uint32_t)x.n, d.predQ, d.posB, d.cellStart,
-                                                    c, x.grids[0], d.grids, d.slotScene, out);
+    k_densities<false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
   return hipGetLastError();
 }
 

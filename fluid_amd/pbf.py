@@ -87,6 +87,18 @@ class Particles:
         else:
             check(lib().pbf_step_dt(self._ctx, float(dt), nsteps), self._ctx)
 
+    def set_schedule(self, schedule: str, block: int = 256) -> None:
+        """'flat', 'graph' (hipGraph-captured step) or 'persistent' (one launch,
+        workgroups run whole scenes); none changes the arithmetic."""
+        check(lib().pbf_set_schedule(self._ctx, _lib.SCHEDULE[schedule], int(block)), self._ctx)
+
+    def step_async(self, nsteps: int = 1, dt: float | None = None) -> None:
+        dt = self.params.dt if dt is None else dt
+        check(lib().pbf_step_async(self._ctx, float(dt), nsteps), self._ctx)
+
+    def sync(self) -> None:
+        check(lib().pbf_sync(self._ctx), self._ctx)
+
     def read(self):
         x = np.empty((self.n, 3), np.float32)
         v = np.empty((self.n, 3), np.float32)

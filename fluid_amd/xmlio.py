@@ -36,7 +36,7 @@ def save_particles(path: str, rest_density: float, x: np.ndarray, v: np.ndarray)
     """Writes the same format (the reference has no writer)."""
     with open(path, "w") as f:
         f.write("<particles>\n  <density>%r</density>\n  <ps>\n" % float(rest_density))
-        for p, q in zip(np.asarray(x, np.float64), np.asarray(v, np.float64)):
+        for p, q in zip(np.asarray(x, np.float64).tolist(), np.asarray(v, np.float64).tolist()):
             f.write("    <particle>\n      <pos>%r %r %r</pos>\n      <v>%r %r %r</v>\n    </particle>\n"
                     % (p[0], p[1], p[2], q[0], q[1], q[2]))
         f.write("  </ps>\n</particles>\n")

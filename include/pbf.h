@@ -100,6 +100,26 @@ PBF_API int pbf_step_dt(pbf_ctx *ctx, float dt, int32_t nsteps);
  * (src/particles.cpp:299-301). */
 PBF_API int pbf_step(pbf_ctx *ctx, int32_t nsteps);
 
+/* Execution schedules (no effect on the arithmetic: every schedule runs the
+ * same per-particle code in the same order and gives the same bits).
+ * No counterpart in the reference (batched extension). */
+enum {
+  PBF_SCHED_FLAT = 0,       /* one launch per pass over all scenes (default) */
+  PBF_SCHED_GRAPH = 1,      /* the flat launches of a step captured once in a
+                               hipGraph and replayed per step */
+  PBF_SCHED_PERSISTENT = 2  /* one launch for all steps: each workgroup runs
+                               whole scenes, largest first, all passes */
+};
+/* Selects the schedule; block is the workgroup size of the persistent
+ * schedule (256, 512 or 1024; ignored otherwise). */
+PBF_API int pbf_set_schedule(pbf_ctx *ctx, int32_t schedule, int32_t block);
+
+/* Enqueues nsteps steps on the context stream and returns without waiting.
+ * Neighbour-list overflow is reported by the next pbf_sync. */
+PBF_API int pbf_step_async(pbf_ctx *ctx, float dt, int32_t nsteps);
+/* Waits for the context stream; reports overflow since the last sync. */
+PBF_API int pbf_sync(pbf_ctx *ctx);
+
 /* Copies the committed state back to the host in load order; any pointer
  * may be NULL.  rho is the SPH density of the last step's XSPH pass (the
  * value the reference keeps in Particle::density, src/particles.cpp:222-231).

@@ -238,6 +238,65 @@ def test_batched_scenes_equal_plain_runs():
         off += n
 
 
+def _batched_items(n=6, nmin=500, nmax=3000):
+    items, p = scenes.batched(nscenes=n, nmin=nmin, nmax=nmax)
+    spec = [(len(x), lo, hi) for x, _, lo, hi in items]
+    x0 = np.concatenate([it[0] for it in items])
+    v0 = np.concatenate([it[1] for it in items])
+    return p, spec, x0, v0
+
+
+@pytest.mark.parametrize("schedule,block", [("graph", 256), ("persistent", 256),
+                                            ("persistent", 512), ("persistent", 1024)])
+def test_schedules_bit_exact_with_flat(schedule, block):
+    p, spec, x0, v0 = _batched_items()
+    with Particles(p, scenes=spec) as f:
+        f.load(x0, v0)
+        f.time_step(nsteps=6)
+        fx, fv, frho = f.read()
+    with Particles(p, scenes=spec) as b:
+        b.set_schedule(schedule, block)
+        b.load(x0, v0)
+        b.time_step(nsteps=6)
+        bx, bv, brho = b.read()
+        assert b.simulate_time == pytest.approx(6 * p.dt, rel=1e-6)
+    assert bits_equal(bx, fx) and bits_equal(bv, fv) and bits_equal(brho, frho)
+
+
+def test_graph_replays_each_match_eager():
+    """Replay the captured step several times with eager work (reads,
+    synchronisations) in between; every replay must equal the eager run."""
+    p, spec, x0, v0 = _batched_items(4)
+    with Particles(p, scenes=spec) as f, Particles(p, scenes=spec) as g:
+        f.load(x0, v0)
+        g.set_schedule("graph")
+        g.load(x0, v0)
+        for _ in range(5):
+            f.time_step()
+            g.time_step()
+            a, b = f.read(), g.read()
+            for u, w in zip(a, b):
+                assert bits_equal(u, w)
+
+
+def test_async_steps_and_overflow_reported_by_sync():
+    sc = scenes.dam_block()
+    with ctx_for(sc.params, sc.x, sc.v) as c:
+        c.step_async(3)
+        c.step_async(2)
+        c.sync()
+        gx, gv, _ = c.read()
+    with ctx_for(sc.params, sc.x, sc.v) as c:
+        c.time_step(nsteps=5)
+        rx, rv, _ = c.read()
+    assert bits_equal(gx, rx) and bits_equal(gv, rv)
+    with ctx_for(sc.params.replace(max_neighbors=4), sc.x, sc.v) as c:
+        c.step_async(1)
+        with pytest.raises(PbfError) as e:
+            c.sync()
+        assert e.value.code == -3
+
+
 def test_error_paths():
     sc = scenes.dam_block()
     with Particles(sc.params) as c:
