@@ -84,6 +84,55 @@ def cpu_baseline(sc, steps: int):
     }
 
 
+def batched_bench(steps: int, looped_steps: int):
+    """512 scenes (counts log-uniform in [1000, 32768]): autotuned batched
+    plan vs the plain path looped scene by scene; a sample of scenes is
+    checked bit for bit between the two."""
+    import numpy as np
+
+    from fluid_amd import batch, scenes
+
+    t0 = time.perf_counter()
+    items, p = scenes.batched()
+    counts = np.array([len(it[0]) for it in items])
+    log(f"batched: {len(items)} scenes, {counts.sum()} particles "
+        f"(min {counts.min()}, max {counts.max()}), generated in {time.perf_counter() - t0:.1f} s")
+    plan, table = batch.autotune(p, items, steps=3, warmup=1, log=log)
+    runner = batch.BatchRunner(p, items, plan)
+    runner.step(1)
+    t0 = time.perf_counter()
+    runner.step(steps)
+    wall_b = time.perf_counter() - t0
+    rate_b = len(items) * steps / wall_b
+    log(f"batched plan {plan.describe()}: {steps} steps in {wall_b:.3f} s, {rate_b:.1f} scene-steps/s")
+    runner.close()
+    # Parity sample: first scenes, batched plan vs looped plain path.
+    sample = items[:4]
+    r = batch.BatchRunner(p, sample, plan)
+    r.step(looped_steps)
+    got = r.read()
+    r.close()
+    wall_l, looped_states = batch.run_looped(p, items, looped_steps)
+    rate_l = len(items) * looped_steps / wall_l
+    exact = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                for g, s in zip(got, looped_states[:4]) for a, b in zip(g, s))
+    log(f"looped plain path: {looped_steps} steps in {wall_l:.3f} s, {rate_l:.1f} scene-steps/s; "
+        f"speedup {rate_b / rate_l:.2f}x, sample bit-exact {exact}")
+    return {
+        "metric": "batched vs looped scene-steps/s",
+        "scenes": len(items),
+        "particles": int(counts.sum()),
+        "plan": plan.describe(),
+        "batched_scene_steps_per_s": rate_b,
+        "batched_steps": steps,
+        "looped_scene_steps_per_s": rate_l,
+        "looped_steps": looped_steps,
+        "speedup": rate_b / rate_l,
+        "sample_bit_exact": exact,
+        "autotune": {pl.describe(): rt for pl, rt in table},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +141,9 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--looped-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a plain 256 MiB device copy (reference for the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -190,6 +242,8 @@ def main() -> int:
         "roofline": roof,
         "passes": passes,
     }
+    if not args.no_batched:
+        out["batched"] = batched_bench(args.batched_steps, args.looped_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -124,7 +124,9 @@ def scene_params(h: float, spacing: float, box_lo, box_hi, dt: float = 0.01,
         box_lo=tuple(float(v) for v in box_lo),
         box_hi=tuple(float(v) for v in box_hi),
         wall_stop_mask=0,
-        max_neighbors=128,
+        # ~30 neighbours at rest; scenes compressed against walls reach 4x
+        # that, and the list costs only memory beyond each particle's count.
+        max_neighbors=256,
     )
     base.update(kw)
     return Params(**base)
