@@ -133,6 +133,43 @@ def batched_bench(steps: int, looped_steps: int):
     }
 
 
+def adjoint_bench():
+    """32^3 block, 60 steps, checkpoints every 10: wall time of the backward
+    pass (recompute included) per step, and of the forward pass."""
+    import numpy as np
+
+    from fluid_amd import scenes
+    from fluid_amd.adjoint import Rollout
+
+    sc, target = scenes.diff_block()
+    r = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=10)
+    r.forward(sc.x, sc.v, sc.params.k_tensile)  # warm-up
+    t0 = time.perf_counter()
+    xT, _ = r.forward(sc.x, sc.v, sc.params.k_tensile)
+    fwd = time.perf_counter() - t0
+    xbar = ((2.0 / sc.n) * (xT - target)).astype(np.float32)
+    t0 = time.perf_counter()
+    _, vb0, kb = r.backward(xbar)
+    bwd = time.perf_counter() - t0
+    ev_ms, contacts = r.stats()
+    r.close()
+    log(f"adjoint {sc.name}: forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms "
+        f"({bwd / sc.steps * 1e3:.3f} ms/step), |dL/dv0| {np.linalg.norm(vb0):.3e}, dL/dk {kb:.4e}")
+    return {
+        "metric": "adjoint ms/step",
+        "workload": sc.name,
+        "particles": sc.n,
+        "steps": sc.steps,
+        "checkpoint_every": 10,
+        "ms_per_step": bwd / sc.steps * 1e3,
+        "ms_per_step_hip_events": ev_ms / sc.steps,
+        "forward_ms_per_step": fwd / sc.steps * 1e3,
+        "wall_contacts": contacts,
+        "grad_norm_v0": float(np.linalg.norm(vb0)),
+        "dL_dk": kb,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +179,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--looped-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
@@ -244,6 +282,8 @@ def main() -> int:
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_steps, args.looped_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -1,0 +1,102 @@
+"""Differentiable PBF rollouts (gradient work only).
+
+:class:`Rollout` wraps the C ABI ``pbf_adj_*`` of include/pbf.h: a forward
+pass of ``nsteps`` steps that keeps a checkpoint every ``checkpoint_every``
+steps, and a backward pass that recomputes each segment from its checkpoint
+(neighbour lists rebuilt) and runs the hand-written adjoint kernels.
+
+:class:`PBFRollout` is the ``torch.autograd.Function`` over it: inputs are
+the initial velocities (n, 3) and the tensile coefficient k (scalar), with
+the initial positions held fixed; outputs are the final positions and
+velocities.  Torch only carries tensors here; all arithmetic is in the HIP
+library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import PbfError, lib
+from .params import Params
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Rollout:
+    def __init__(self, params: Params, n: int, nsteps: int, checkpoint_every: int = 10):
+        self.params = params
+        self.n = int(n)
+        self.nsteps = int(nsteps)
+        self._h = C.c_void_p()
+        cp = params.to_c()
+        self._check(lib().pbf_adj_create(C.byref(cp), self.n, self.nsteps, int(checkpoint_every),
+                                         C.byref(self._h)))
+
+    def _check(self, rc: int) -> None:
+        if rc < 0:
+            msg = lib().pbf_adj_last_error(self._h if self._h else None)
+            raise PbfError(rc, msg.decode() if msg else "")
+
+    def forward(self, x0, v0, k: float):
+        x0 = np.ascontiguousarray(x0, np.float32).reshape(-1, 3)
+        v0 = np.ascontiguousarray(v0, np.float32).reshape(-1, 3)
+        xT = np.empty_like(x0)
+        vT = np.empty_like(v0)
+        self._check(lib().pbf_adj_forward(self._h, _f(x0), _f(v0), float(k), _f(xT), _f(vT)))
+        return xT, vT
+
+    def backward(self, xbar_T, vbar_T=None):
+        xb = np.ascontiguousarray(xbar_T, np.float32).reshape(-1, 3)
+        vb = None if vbar_T is None else np.ascontiguousarray(vbar_T, np.float32).reshape(-1, 3)
+        xb0 = np.empty_like(xb)
+        vb0 = np.empty_like(xb)
+        kb = C.c_double()
+        self._check(lib().pbf_adj_backward(self._h, _f(xb), None if vb is None else _f(vb),
+                                           _f(xb0), _f(vb0), C.byref(kb)))
+        return xb0, vb0, kb.value
+
+    def stats(self):
+        ms, contacts = C.c_double(), C.c_int64()
+        self._check(lib().pbf_adj_stats(self._h, C.byref(ms), C.byref(contacts)))
+        return ms.value, contacts.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().pbf_adj_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rollout_function():
+    """Returns the torch.autograd.Function class (torch imported lazily)."""
+    import torch
+
+    class PBFRollout(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, v0, k, x0, rollout):
+            xT, vT = rollout.forward(x0.detach().cpu().numpy(), v0.detach().cpu().numpy(),
+                                     float(k.detach().cpu()))
+            ctx.rollout = rollout
+            dev = v0.device
+            return torch.from_numpy(xT).to(dev), torch.from_numpy(vT).to(dev)
+
+        @staticmethod
+        def backward(ctx, gx, gv):
+            gxn = gx.detach().cpu().numpy() if gx is not None else None
+            gvn = gv.detach().cpu().numpy() if gv is not None else None
+            if gxn is None:
+                gxn = np.zeros((ctx.rollout.n, 3), np.float32)
+            xb0, vb0, kb = ctx.rollout.backward(gxn, gvn)
+            dev = gx.device if gx is not None else gv.device
+            return (torch.from_numpy(vb0).to(dev), torch.tensor(kb, dtype=torch.float32, device=dev),
+                    torch.from_numpy(xb0).to(dev), None)
+
+    return PBFRollout

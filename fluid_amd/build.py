@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpbf_hip.so")
 DRIVER = os.path.join(PKG, "host", "pbf_driver")
 
-HIP_SOURCES = ["pbf_kernels.hip", "pbf_capi.hip"]
+HIP_SOURCES = ["pbf_kernels.hip", "pbf_capi.hip", "pbf_adjoint.hip"]
 
 # IEEE single precision in source order on the device: no FMA contraction,
 # correctly rounded division and square root.  Packed FP32 instructions are

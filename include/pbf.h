@@ -140,6 +140,30 @@ PBF_API int64_t pbf_count(const pbf_ctx *ctx);
  * when ctx is NULL).  Never NULL. */
 PBF_API const char *pbf_last_error(const pbf_ctx *ctx);
 
+/* ---- differentiable extension (no counterpart in the reference) --------
+ * Reverse-mode derivatives of nsteps steps of one scene (collision treated
+ * as the identity; pbf_adj_stats counts wall contacts).  The forward pass
+ * keeps (x, v) every checkpoint_every steps; the backward pass recomputes
+ * each segment from its checkpoint, rebuilding the neighbour lists, and runs
+ * hand-written adjoint kernels of the lambda, delta-p, XSPH and vorticity
+ * passes. */
+typedef struct pbf_adj pbf_adj;
+PBF_API int pbf_adj_create(const pbf_params *p, int64_t n, int32_t nsteps,
+                           int32_t checkpoint_every, pbf_adj **out);
+PBF_API int pbf_adj_destroy(pbf_adj *adj);
+/* Runs the steps from (x0, v0) with tensile coefficient k_tensile; writes the
+ * final state (xT, vT may be NULL).  n*3 floats each, load order. */
+PBF_API int pbf_adj_forward(pbf_adj *adj, const float *x0, const float *v0, float k_tensile,
+                            float *xT, float *vT);
+/* Given dL/dx_T (and dL/dv_T, may be NULL), writes dL/dx_0, dL/dv_0 (n*3
+ * each, may be NULL) and dL/dk. */
+PBF_API int pbf_adj_backward(pbf_adj *adj, const float *xbar_T, const float *vbar_T,
+                             float *xbar_0, float *vbar_0, double *kbar);
+/* Wall time of the last backward pass (HIP events, ms) and the number of
+ * particle-iterations found on a wall during it. */
+PBF_API int pbf_adj_stats(const pbf_adj *adj, double *backward_ms, int64_t *wall_contacts);
+PBF_API const char *pbf_adj_last_error(const pbf_adj *adj);
+
 /* ---- test hooks (inspection of the last step; no counterpart) ---------- */
 
 /* Neighbour lists built by the last step, in load order: counts[n]; ids
