@@ -555,8 +555,7 @@ PBF_API int pbf_adj_forward(pbf_adj* a, const float* x0, const float* v0, float 
   a->k = k_tensile;
   int rc = pbf_load(a->ctx, x0, v0, a->n);
   if (rc != PBF_OK) return afail(a, rc, pbf_last_error(a->ctx));
-  pbf::SimConst k;
-  pbf::derive_constants(c->p, c->p.dt, &k);
+  const pbf::SimConst k = pbf::consts_for(*c, c->p.dt);
   for (int t = 0; t < a->nsteps; t++) {
     if (t % a->every == 0) {
       const size_t o = (size_t)(t / a->every) * (size_t)a->npad;
@@ -594,8 +593,7 @@ PBF_API int pbf_adj_backward(pbf_adj* a, const float* xbar_T, const float* vbar_
   AHIP(a, hipMemcpyAsync(a->xbar, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
   AHIP(a, hipMemcpyAsync(a->vbar, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
   AHIP(a, pbf::launch_fill_u32(a->contacts, 1, 0u, s));
-  pbf::SimConst k;
-  pbf::derive_constants(c->p, c->p.dt, &k);
+  const pbf::SimConst k = pbf::consts_for(*c, c->p.dt);
   int rc = PBF_OK;
   for (int seg = a->nckpt - 1; seg >= 0; seg--) {
     const int t0 = seg * a->every;

@@ -6,6 +6,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -131,6 +133,58 @@ int ensure_device(Ctx* x) {
   return PBF_OK;
 }
 
+// Verdicts of the exhaustive device check of the short square root,
+// reciprocal and division sequences, per device (and per h^9 for the
+// division).  A failed or skipped check leaves the IEEE expansions on.
+struct MathVerdict {
+  bool checked = false;
+  int rcp = 0, sqrt = 0;
+  std::map<uint64_t, int> div;  // keyed by the bits of h^9 and the lowest binade
+};
+std::mutex g_math_mu;
+std::map<int, MathVerdict> g_math;
+
+int verify_math(Ctx* x) {
+  if (getenv("PBF_IEEE_MATH")) return PBF_OK;  // keep the IEEE expansions
+  const SimConst k = pbf::consts_for(*x, x->p.dt);
+  // The poly6 numerator c1 (h^2 - r^2)^3 is 0 or lies in [amin, amax]:
+  // h^2 - r^2 >= h^2 - pred(h^2) for every float r^2 < h^2, and rounding is
+  // monotonic.
+  const float t = k.h2, amax = k.c1 * ((t * t) * t);
+  const float tmin = k.h2 - nextafterf(k.h2, 0.0f), amin = k.c1 * ((tmin * tmin) * tmin);
+  const bool div_range = amin >= 0x1p-126f && amax < k.div_hi;
+  const int e_hi = ilogbf(k.div_hi), e_lo = div_range ? ilogbf(amin) : e_hi;
+  uint32_t hb9;
+  memcpy(&hb9, &k.h9, sizeof hb9);
+  const uint64_t key = ((uint64_t)hb9 << 32) | (uint32_t)(e_lo + 1024);
+  std::lock_guard<std::mutex> lock(g_math_mu);
+  MathVerdict& v = g_math[x->device];
+  auto it = v.div.find(key);
+  if (!v.checked || it == v.div.end()) {
+    uint32_t* bad = nullptr;
+    HIPCHK(x, hipMalloc((void**)&bad, 4 * sizeof(uint32_t)));
+    uint32_t hb[4] = {1, 1, 1, 1};
+    hipError_t e = pbf::launch_fill_u32(bad, 4, 0u, x->stream);
+    if (e == hipSuccess) e = pbf::launch_verify_math(k.h9, k.r9, e_lo, e_hi, bad, x->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hb, bad, sizeof hb, hipMemcpyDeviceToHost, x->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+    (void)hipFree(bad);
+    HIPCHK(x, e);
+    v.checked = true;
+    v.rcp = hb[1] == 0;
+    v.sqrt = hb[2] == 0;
+    it = v.div.emplace(key, e_lo < e_hi && hb[0] == 0).first;
+  }
+  // The arguments the kernels pass must lie in the checked ranges: spiky's
+  // reciprocal argument is h^6 |r| with EPS_D <= |r| < h (or 1), and square
+  // roots below 2^-47 are only compared with EPS_D.
+  const float zlo = k.h6 * k.eps_d, zhi = k.h6 * k.h;
+  x->fast_div = it->second && div_range;
+  x->fast_rcp = v.rcp && zlo >= 0x1p-100f && zhi < 0x1p100f;
+  x->fast_sqrt = v.sqrt && k.eps_d >= 0x1p-47f;
+  return PBF_OK;
+}
+
 int finish_create(Ctx* x, pbf_ctx** out) {
   uint64_t total = 0;
   for (auto& g : x->grids) {
@@ -154,6 +208,12 @@ int finish_create(Ctx* x, pbf_ctx** out) {
     return fail(nullptr, PBF_ERR_HIP, m);
   }
   x->device = dev;
+  if (verify_math(x) != PBF_OK) {
+    std::string m = x->err;
+    (void)hipStreamDestroy(x->stream);
+    delete x;
+    return fail(nullptr, PBF_ERR_HIP, m);
+  }
   *out = reinterpret_cast<pbf_ctx*>(x);
   return PBF_OK;
 }
@@ -282,6 +342,25 @@ void derive_constants(const pbf_params& p, float dt, SimConst* c) {
   c->inv_dt = 1.0f / dt;
   c->gdt = p.gravity * dt;
   c->n_tensile = p.n_tensile;
+  // Short division by h^9: numerators below div_hi keep a * (1/h^9) below
+  // 2^100.  The flags stay off until a device has checked the sequences
+  // (consts_for).
+  c->r9 = 1.0f / h9;
+  int e_hi = 100;
+  if (c->r9 > 0.0f && isfinite(c->r9)) e_hi = std::min(100, 100 - ilogbf(c->r9) - 1);
+  c->div_hi = ldexpf(1.0f, std::max(e_hi, -126));
+  c->fast_div = 0;
+  c->fast_rcp = 0;
+  c->fast_sqrt = 0;
+}
+
+SimConst consts_for(const Ctx& x, float dt) {
+  SimConst k;
+  derive_constants(x.p, dt, &k);
+  k.fast_div = x.fast_div;
+  k.fast_rcp = x.fast_rcp;
+  k.fast_sqrt = x.fast_sqrt;
+  return k;
 }
 
 void derive_grid(const pbf_params& p, const float* lo, const float* hi, SceneGrid* g) {
@@ -486,8 +565,7 @@ int sync_check(Ctx* x) {
 // Enqueues nsteps steps with the context's schedule.
 int enqueue_steps(Ctx* x, float dt, int nsteps) {
   if (nsteps == 0) return PBF_OK;
-  SimConst k;
-  pbf::derive_constants(x->p, dt, &k);
+  const SimConst k = pbf::consts_for(*x, dt);
   if (x->schedule == PBF_SCHED_PERSISTENT) {
     const int per = pbf::persistent_max_blocks_per_cu(x->pblock);
     const int grid = std::max(1, std::min(x->nscenes, std::max(1, x->ncu) * per));
@@ -575,8 +653,7 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
   if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
   if (!is_finite_f(dt) || !(dt > 0.0f) || nsteps < 0 || !pass_ms || npass < PBF_NPASS)
     return fail(x, PBF_ERR_INVALID, "dt > 0, nsteps >= 0, pass_ms[PBF_NPASS] required");
-  SimConst k;
-  pbf::derive_constants(x->p, dt, &k);
+  const SimConst k = pbf::consts_for(*x, dt);
   for (int i = 0; i < npass; i++) {
     pass_ms[i] = 0.0f;
     if (pass_launches) pass_launches[i] = 0;
@@ -617,8 +694,7 @@ PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
   if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
   if (!is_finite_f(dt) || !(dt > 0.0f) || stage < PBF_STAGE_PREDICT || stage > PBF_STAGE_XSPH)
     return fail(x, PBF_ERR_INVALID, "dt must be > 0 and stage a PBF_STAGE_* value");
-  SimConst k;
-  pbf::derive_constants(x->p, dt, &k);
+  const SimConst k = pbf::consts_for(*x, dt);
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
   drop_graph(x);
   x->debug_dirty = true;
@@ -655,8 +731,7 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
   if (rc != PBF_OK) return rc;
   if (!rho) return fail(x, PBF_ERR_INVALID, "rho is NULL");
   if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
-  SimConst k;
-  pbf::derive_constants(x->p, x->p.dt, &k);
+  const SimConst k = pbf::consts_for(*x, x->p.dt);
   HIPCHK(x, pbf::launch_densities(*x, k, x->d.stage));
   if ((rc = copy_out(x, x->d.stage, rho, (size_t)x->n * sizeof(float))) != PBF_OK) return rc;
   // The density pass re-sorted the committed state into posB/velB.
@@ -769,15 +844,18 @@ PBF_API int pbf_debug_read(pbf_ctx* c, int32_t which, float* out4) {
 PBF_API int pbf_debug_derived(const pbf_ctx* c, float* out, int32_t cap) {
   const Ctx* x = C(c);
   if (!x || !out || cap < 25) return fail(const_cast<Ctx*>(x), PBF_ERR_INVALID, "cap >= 25");
-  SimConst k;
-  pbf::derive_constants(x->p, x->p.dt, &k);
+  const SimConst k = pbf::consts_for(*x, x->p.dt);
   const SceneGrid& g = x->grids[0];
   const float vals[25] = {k.h,   k.h2,   k.h6,     k.h9,   k.c1,  k.m3c2, k.S,
                           k.inv_rho0, k.rho0, k.eps, k.negk, k.xc, k.dtv,  k.inv_dt,
                           k.dt,  k.gdt,  x->p.h * x->p.cell_scale, g.inv_cell, k.eps_d,
                           g.lo_e[0], g.lo_e[1], g.lo_e[2], g.hi_e[0], g.hi_e[1], g.hi_e[2]};
   memcpy(out, vals, sizeof vals);
-  return 25;
+  if (cap < 28) return 25;
+  out[25] = (float)k.fast_div;
+  out[26] = (float)k.fast_rcp;
+  out[27] = (float)k.fast_sqrt;
+  return 28;
 }
 
 }  // extern "C"

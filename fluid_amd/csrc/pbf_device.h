@@ -27,6 +27,11 @@ struct SimConst {
   float gdt;             // g * dt
   float eps_d;           // EPS_D
   int n_tensile;         // N
+  float r9;              // 1/h^9 rounded to nearest (for div_seq)
+  float div_hi;          // div_seq is checked for numerators below div_hi
+  int fast_div;          // poly6 divides by h^9 with div_seq
+  int fast_rcp;          // spiky takes its reciprocal with rcp_seq
+  int fast_sqrt;         // spiky takes its square root with sqrt_rn
 };
 
 // Grid and collision box of one scene.
@@ -205,24 +210,77 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
-// Branch-free forms of poly6 / spiky for the neighbour loops: the same
-// expressions evaluated unconditionally, with the reference's cut-offs
-// applied by selection (identical results, no divergent branches).
+// ---- correctly rounded square root, reciprocal and division -------------
+// The neighbour loops need IEEE-rounded sqrt and division (the CPU path does
+// them in IEEE single precision).  These sequences give the same bits with
+// fewer instructions than the general expansions.  Each is used only when a
+// flag of SimConst says that an exhaustive check on the device (every
+// significand of every binade of the stated range) matched the IEEE result
+// and the host has shown that every argument the kernels pass lies in that
+// range; otherwise the IEEE expansion runs.  The flags are kernel arguments,
+// so the choice is a uniform branch.
+//
+// sqrt_rn: the hardware square root (within 1 ulp) corrected by the sign of
+//   the residuals x - s'*s for the neighbours s' = s -+ 1 ulp.  Checked on 0
+//   and [2^-96, 2^128).  Below 2^-96 (no pre-scaling) the root may be off by
+//   an ulp; those roots are below 2^-47 and the callers only compare them
+//   with EPS_D, which the host requires to be at least 2^-47.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sdn = __int_as_float(__float_as_int(s) - 1);
+  const float sup = __int_as_float(__float_as_int(s) + 1);
+  const float rdn = fmaf(-sdn, s, x);
+  const float rup = fmaf(-sup, s, x);
+  const float t = (rdn <= 0.0f) ? sdn : s;
+  return (rup > 0.0f) ? sup : t;
+}
+
+// rcp_seq: hardware reciprocal refined by two Newton steps with FMA.
+//   Checked on [2^-100, 2^100]; the spiky kernel passes h^6 |r| with
+//   EPS_D <= |r| < h, or 1.
+__device__ __forceinline__ float rcp_seq(float z) {
+  float y = __builtin_amdgcn_rcpf(z);
+  float e = fmaf(-z, y, 1.0f);
+  y = fmaf(e, y, y);
+  e = fmaf(-z, y, 1.0f);
+  return fmaf(e, y, y);
+}
+
+// div_seq: a / b as a * RN(1/b) with two FMA residual corrections.
+//   Checked for b = h^9 on 0 and on the binades from that of amin up to
+//   div_hi, where the poly6 numerator c1 (h^2 - r^2)^3 is 0 or lies in
+//   [amin, amax] (amin from the smallest positive h^2 - r^2); the host
+//   requires amin to be normal and amax below div_hi.
+__device__ __forceinline__ float div_seq(float a, float b, float rb) {
+  float q = a * rb;
+  float e = fmaf(-q, b, a);
+  q = fmaf(e, rb, q);
+  e = fmaf(-q, b, a);
+  return fmaf(e, rb, q);
+}
+
+// poly6 / spiky for the neighbour loops: the same expressions as poly6() and
+// spiky(), with the reference's cut-offs applied by selection (no divergent
+// branches).  Outside the cut-off poly6 divides 0 (giving +0) and spiky
+// scales by a zero coefficient, giving a +-0 gradient: the callers only add
+// these terms to sums that start at +0, where adding +-0 changes no bit
+// (a round-to-nearest sum is -0 only when both terms are -0).
 __device__ __forceinline__ float poly6_sel(const SimConst& c, float r2) {
   const float t = c.h2 - r2;
-  const float w = (c.c1 * ((t * t) * t)) / c.h9;
-  return (r2 >= c.h2) ? 0.0f : w;
+  const float a = (r2 >= c.h2) ? 0.0f : c.c1 * ((t * t) * t);
+  if (c.fast_div) return div_seq(a, c.h9, c.r9);
+  return a / c.h9;
 }
 
 __device__ __forceinline__ float3 spiky_sel(const SimConst& c, float rx, float ry, float rz,
                                             float r2) {
-  const float rl = sqrtf(r2);
-  const float q = c.h - rl;
-  const float coef = c.m3c2 * (q * q);
-  const float rc = 1.0f / (c.h6 * rl);
+  const float rl = c.fast_sqrt ? sqrt_rn(r2) : sqrtf(r2);
   const bool zero = (rl >= c.h) || (rl < c.eps_d);
-  return make_float3(zero ? 0.0f : rc * (coef * rx), zero ? 0.0f : rc * (coef * ry),
-                     zero ? 0.0f : rc * (coef * rz));
+  const float q = c.h - rl;
+  const float coef = zero ? 0.0f : c.m3c2 * (q * q);
+  const float z = zero ? 1.0f : c.h6 * rl;
+  const float rc = c.fast_rcp ? rcp_seq(z) : 1.0f / z;
+  return make_float3(rc * (coef * rx), rc * (coef * ry), rc * (coef * rz));
 }
 
 // Visits the neighbours of particle i in list order.  Groups of U
@@ -253,6 +311,71 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t i, int cnt, int K4,
     const uint32_t j = nbr[nbr_slot(i, k, K4)];
     body(j, load(j));
   }
+}
+
+// The same visit, software-pipelined over 4-entry chunks: while the bodies
+// of chunk c run, the gathers of chunk c+1 and the index load of chunk c+2
+// are in flight, so a wave waits for one memory round trip per chunk at most
+// instead of two.  The partial last chunk comes from the same pipeline;
+// its gathers and bodies are predicated.
+#ifndef PBF_PIPE
+#define PBF_PIPE 1
+#endif
+template <class Load, class Body>
+__device__ __forceinline__ void for_each_neighbor_pipe(uint32_t i, int cnt, int K4,
+                                                       const uint32_t* __restrict__ nbr,
+                                                       Load load, Body body) {
+  using D = decltype(load(0u));
+  auto idx = [&](int ch) {
+    const uint4 q = *reinterpret_cast<const uint4*>(nbr + nbr_chunk(i, ch, K4));
+    return q;
+  };
+  auto gather = [&](const uint4& q, D* d) {
+    d[0] = load(q.x);
+    d[1] = load(q.y);
+    d[2] = load(q.z);
+    d[3] = load(q.w);
+  };
+  const int nfull = cnt >> 2, nch = (cnt + 3) >> 2;
+  uint4 a = make_uint4(i, i, i, i), an = a;
+  D d[4];
+  if (nch > 0) a = idx(0);
+  if (nfull > 0) gather(a, d);
+  if (nch > 1) an = idx(1);
+  for (int ch = 0; ch < nfull; ch++) {
+    D dn[4];
+    uint4 ann = an;
+    if (ch + 1 < nfull) gather(an, dn);
+    if (ch + 2 < nch) ann = idx(ch + 2);
+    body(a.x, d[0]);
+    body(a.y, d[1]);
+    body(a.z, d[2]);
+    body(a.w, d[3]);
+    a = an;
+    an = ann;
+#pragma unroll
+    for (int u = 0; u < 4; u++) d[u] = dn[u];
+  }
+  const int rem = cnt & 3;
+  if (rem > 0) {
+    D t0 = load(a.x), t1, t2;
+    if (rem > 1) t1 = load(a.y);
+    if (rem > 2) t2 = load(a.z);
+    body(a.x, t0);
+    if (rem > 1) body(a.y, t1);
+    if (rem > 2) body(a.z, t2);
+  }
+}
+
+// Chooses the visit of the lists: pipelined or grouped by U.
+template <int U, class Load, class Body>
+__device__ __forceinline__ void visit_neighbors(uint32_t i, int cnt, int K4,
+                                                const uint32_t* __restrict__ nbr, Load load,
+                                                Body body) {
+  if (PBF_PIPE && U == 4)
+    for_each_neighbor_pipe(i, cnt, K4, nbr, load, body);
+  else
+    for_each_neighbor<U>(i, cnt, K4, nbr, load, body);
 }
 
 }  // namespace pbf

@@ -73,6 +73,9 @@ struct Ctx {
   int schedule = PBF_SCHED_FLAT;     // PBF_SCHED_*
   int pblock = 256;                  // workgroup size of the persistent schedule
   int ncu = 0;                       // compute units of the device
+  int fast_div = 0;                  // short division by h^9 verified exact
+  int fast_rcp = 0;                  // short reciprocal verified exact
+  int fast_sqrt = 0;                 // short square root verified exact
   hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)
   hipGraphExec_t graph_exec = nullptr;
   float graph_dt = 0.0f;
@@ -105,6 +108,10 @@ hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, 
 hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride);
 hipError_t launch_densities(Ctx& x, const SimConst& c, float* out);
 hipError_t launch_persistent(Ctx& x, const SimConst& c, int nsteps, int block, int grid);
+hipError_t launch_verify_math(float b, float rb, int div_e_lo, int div_e_hi, uint32_t* bad,
+                              hipStream_t s);
+// Step constants of a context, with its verified short-division flags.
+SimConst consts_for(const Ctx& x, float dt);
 int persistent_max_blocks_per_cu(int block);
 hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx);
 hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,

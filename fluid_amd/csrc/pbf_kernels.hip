@@ -267,61 +267,116 @@ __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d) {
   d.cellSorted[dst] = c;
 }
 
+// Distance from coordinate x to the near face of cell k+side of the axis
+// (side -1: lower face of cell k, +1: upper face), never negative.
+__device__ __forceinline__ float face_gap(float x, float lo, float cell, int k, int side) {
+  if (side == 0) return 0.0f;
+  const float f = lo + (float)(side < 0 ? k : k + 1) * cell;
+  // The face is within a few ulps of the coordinates involved of where the
+  // cell assignment puts it; tol bounds that generously.
+  const float tol = (fabsf(x) + fabsf(lo) + fabsf(f)) * 0x1p-20f;
+  const float gap = (side < 0 ? x - f : f - x) - tol;
+  return gap > 0.0f ? gap : 0.0f;
+}
+
 // Neighbour sets of src/particles.cpp:258-265 (|x*_i - x*_j|^2 <= H2, j != i),
-// found in the 27 cells around i.  For each (dz, dy) the three cells dx =
-// -1..1 are contiguous in sorted order, so the search walks 9 index ranges in
-// ascending order.  Found indices collect in a 4-entry register chunk that
-// is stored whole (one 16-byte store) when full.
+// found in the 27 cells around i.  For each (dz, dy) the cells dx = -1..1 are
+// contiguous in sorted order, so the search walks at most 9 index ranges in
+// ascending order.  Rows and end cells whose box lies farther than 1.01 h
+// from the particle are skipped (they hold no neighbour; the 1% margin covers
+// the rounding of the cell faces), which leaves the list and its order
+// unchanged.  Candidates are tested 32 at a time, branch-free, into a hit
+// mask (self cleared afterwards); the hits are then appended in order.
+#ifndef PBF_NBR_PUSH
+#define PBF_NBR_PUSH 0  // 1: one 4-byte store per hit; 0: 16-byte chunk stores
+#endif
 __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, const SceneGrid& g) {
   const float4 xi = d.predQ[i];
   int cc[3];
   cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
-  const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
-  const int x1 = cc[0] + 1 < g.dims[0] ? cc[0] + 1 : g.dims[0] - 1;
+  const float cell = 1.0f / g.inv_cell;
+  const float hm2 = h2 * 1.0201f;  // (1.01 h)^2
   const float4* __restrict__ pred = d.predQ;
+  const uint32_t cap = (uint32_t)d.K4 * kChunk;
   uint32_t cnt = 0;
+#if PBF_NBR_PUSH
+  // Slot k of particle i: chunk k/4 (chunks 64*4 entries apart), entry k%4.
+  uint32_t* const base = d.nbr + nbr_chunk(i, 0, d.K4);
+  auto push = [&](uint32_t j) {
+    if (cnt < cap) base[(size_t)(cnt >> 2) * (64u * kChunk) + (cnt & 3u)] = j;
+    cnt++;
+  };
+#else
   uint32_t buf[kChunk];
 #pragma unroll
   for (int u = 0; u < kChunk; u++) buf[u] = 0u;
-  auto flush = [&](int chunk) {
-    if (chunk < d.K4)
-      *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
-          make_uint4(buf[0], buf[1], buf[2], buf[3]);
-  };
-  auto test = [&](uint32_t j, const float4& xj) {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float d2 = rx * rx + ry * ry + rz * rz;
-    if (d2 <= h2 && j != i) {
-      const uint32_t slot = cnt & 3u;
+  auto push = [&](uint32_t j) {
+    const uint32_t slot = cnt & 3u;
 #pragma unroll
-      for (int u = 0; u < kChunk; u++) buf[u] = (slot == (uint32_t)u) ? j : buf[u];
-      cnt++;
-      if ((cnt & 3u) == 0u) flush((int)(cnt >> 2) - 1);
+    for (int u = 0; u < kChunk; u++) buf[u] = (slot == (uint32_t)u) ? j : buf[u];
+    cnt++;
+    if ((cnt & 3u) == 0u) {
+      const int chunk = (int)(cnt >> 2) - 1;
+      if (chunk < d.K4)
+        *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
+            make_uint4(buf[0], buf[1], buf[2], buf[3]);
     }
   };
+#endif
+  auto hit = [&](const float4& xj) -> uint32_t {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    return d2 <= h2 ? 1u : 0u;
+  };
+  const float gxl = face_gap(xi.x, g.lo[0], cell, cc[0], -1);
+  const float gxr = face_gap(xi.x, g.lo[0], cell, cc[0], 1);
   for (int dz = -1; dz <= 1; dz++) {
     const int z = cc[2] + dz;
     if (z < 0 || z >= g.dims[2]) continue;
+    const float gz = face_gap(xi.z, g.lo[2], cell, cc[2], dz);
     for (int dy = -1; dy <= 1; dy++) {
       const int y = cc[1] + dy;
       if (y < 0 || y >= g.dims[1]) continue;
+      const float gy = face_gap(xi.y, g.lo[1], cell, cc[1], dy);
+      const float yz2 = gy * gy + gz * gz;
+      if (yz2 > hm2) continue;
+      const int x0 = (cc[0] > 0 && gxl * gxl + yz2 <= hm2) ? cc[0] - 1 : cc[0];
+      const int x1 = (cc[0] + 1 < g.dims[0] && gxr * gxr + yz2 <= hm2) ? cc[0] + 1 : cc[0];
       const uint32_t row =
           g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
       const uint32_t s = d.cellStart[row + (uint32_t)x0];
       const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
-      uint32_t j = s;
-      for (; j + 4 <= e; j += 4) {
-        const float4 a = pred[j], b = pred[j + 1], c4 = pred[j + 2], q = pred[j + 3];
-        test(j, a);
-        test(j + 1, b);
-        test(j + 2, c4);
-        test(j + 3, q);
+      for (uint32_t jb = s; jb < e; jb += 32) {
+        const uint32_t je = min(e, jb + 32u), last = je - 1u;
+        uint32_t mask = 0;
+        for (uint32_t j = jb; j < je; j += 4) {
+          // Loads past the range repeat the last candidate; their bits are
+          // masked off.
+          const float4 a = pred[j], b = pred[min(j + 1u, last)], c4 = pred[min(j + 2u, last)],
+                       q = pred[min(j + 3u, last)];
+          const uint32_t m4 = hit(a) | (hit(b) << 1) | (hit(c4) << 2) | (hit(q) << 3);
+          const uint32_t left = je - j;
+          const uint32_t valid = left >= 4u ? 0xFu : ((1u << left) - 1u);
+          mask |= (m4 & valid) << (j - jb);
+        }
+        const uint32_t self = i - jb;
+        if (self < 32u) mask &= ~(1u << self);
+        while (mask) {
+          const uint32_t u = (uint32_t)__builtin_ctz(mask);
+          mask &= mask - 1u;
+          push(jb + u);
+        }
       }
-      for (; j < e; j++) test(j, pred[j]);
     }
   }
-  if (cnt & 3u) flush((int)(cnt >> 2));
-  const uint32_t cap = (uint32_t)d.K4 * kChunk;
+#if !PBF_NBR_PUSH
+  if (cnt & 3u) {
+    const int chunk = (int)(cnt >> 2);
+    if (chunk < d.K4)
+      *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
+          make_uint4(buf[0], buf[1], buf[2], buf[3]);
+  }
+#endif
   if (cnt > cap) {
     atomicMax(&d.flags[0], cnt);
     cnt = cap;
@@ -336,7 +391,7 @@ __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimCo
   const float4 xi = in[i];
   const int cnt = (int)d.ncnt[i];
   float dens = 0.0f, denom = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
-  for_each_neighbor<kUnroll>(
+  visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -368,7 +423,7 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   const float4 xi = in[i];
   const int cnt = (int)d.ncnt[i];
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  for_each_neighbor<kUnroll>(
+  visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -407,7 +462,7 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
   struct XV {
     float4 x, v;
   };
-  for_each_neighbor<kUnrollXsph>(
+  visit_neighbors<kUnrollXsph>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{in[j], vel[j]}; },
       [&](uint32_t, const XV& q) {
         const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
@@ -436,7 +491,7 @@ __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimCons
   const float4 xi = in[i];
   const int cnt = (int)d.ncnt[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  for_each_neighbor<kUnroll>(
+  visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -664,9 +719,45 @@ __global__ __launch_bounds__(kBlock) void k_nbr_stats(uint32_t n, const uint32_t
   atomicMax(mx, m);
 }
 
+// Exhaustive check of the short division, reciprocal and square root
+// sequences against the IEEE operations: thread m takes significand m and
+// runs it through every binade of the checked ranges.  bad[0..2] count the
+// mismatches of the division by b (0 and [2^div_e_lo, 2^div_e_hi)), the
+// reciprocal ([2^-100, 2^100)) and the square root (0 and [2^-96, 2^128)).
+__global__ __launch_bounds__(kBlock) void k_verify_math(float b, float rb, int div_e_lo,
+                                                        int div_e_hi, uint32_t* __restrict__ bad) {
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= (1u << 23)) return;
+  auto differ = [](float u, float v) { return __float_as_uint(u) != __float_as_uint(v) ? 1u : 0u; };
+  uint32_t nd = 0, nr = 0, ns = 0;
+  if (m == 0) nd += differ(div_seq(0.0f, b, rb), 0.0f / b);
+  for (int e = div_e_lo; e < div_e_hi; e++) {
+    const float a = __uint_as_float(((uint32_t)(e + 127) << 23) | m);
+    nd += differ(div_seq(a, b, rb), a / b);
+  }
+  for (int e = -100; e < 100; e++) {
+    const float z = __uint_as_float(((uint32_t)(e + 127) << 23) | m);
+    nr += differ(rcp_seq(z), 1.0f / z);
+  }
+  if (m == 0) ns += differ(sqrt_rn(0.0f), sqrtf(0.0f));
+  for (int e = -96; e < 128; e++) {
+    const float x = __uint_as_float(((uint32_t)(e + 127) << 23) | m);
+    ns += differ(sqrt_rn(x), sqrtf(x));
+  }
+  if (nd) atomicAdd(bad, nd);
+  if (nr) atomicAdd(bad + 1, nr);
+  if (ns) atomicAdd(bad + 2, ns);
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
+
+hipError_t launch_verify_math(float b, float rb, int div_e_lo, int div_e_hi, uint32_t* bad,
+                              hipStream_t s) {
+  k_verify_math<<<(1u << 23) / kBlock, kBlock, 0, s>>>(b, rb, div_e_lo, div_e_hi, bad);
+  return hipGetLastError();
+}
 
 // ------------------------------------------------------------ launchers --
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {

@@ -160,8 +160,15 @@ class Particles:
 
     def derived(self) -> np.ndarray:
         out = np.empty(32, np.float32)
-        k = check(lib().pbf_debug_derived(self._ctx, _fptr(out), 32), self._ctx)
-        return out[:k]
+        check(lib().pbf_debug_derived(self._ctx, _fptr(out), 25), self._ctx)
+        return out[:25]
+
+    def math_flags(self) -> dict:
+        """Which short sequences (division by h^9, reciprocal, square root)
+        passed the exhaustive device check and are in use."""
+        out = np.empty(32, np.float32)
+        check(lib().pbf_debug_derived(self._ctx, _fptr(out), 32), self._ctx)
+        return {"div": bool(out[25]), "rcp": bool(out[26]), "sqrt": bool(out[27])}
 
 
 PbfError = _lib.PbfError

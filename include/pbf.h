@@ -224,7 +224,11 @@ PBF_API int pbf_step_profiled(pbf_ctx *ctx, float dt, int32_t nsteps, float *pas
                               int32_t *pass_launches, int32_t npass);
 
 /* Derived constants in float, same order as the oracle's ORC_* indices
- * (h, h2, h^6, h^9, ...), and the number written. */
+ * (h, h2, h^6, h^9, ...), and the number written (25).  With cap >= 28 it
+ * also writes, as 0/1, whether the short division by h^9, reciprocal and
+ * square root sequences passed the exhaustive device check at create time
+ * and are in use (28).  Setting PBF_IEEE_MATH in the environment before
+ * pbf_create keeps the IEEE expansions; both give the same bits. */
 PBF_API int pbf_debug_derived(const pbf_ctx *ctx, float *out, int32_t cap);
 
 #ifdef __cplusplus

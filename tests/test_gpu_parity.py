@@ -210,6 +210,33 @@ def test_two_sphere_collision_bit_exact(orc):
     assert bits_equal(gx, xg) and bits_equal(gv, vg)
 
 
+def test_short_math_sequences_verified_and_same_bits(orc):
+    """The short division/reciprocal/square-root sequences pass the device's
+    exhaustive check at create time, and a context that keeps the IEEE
+    expansions (PBF_IEEE_MATH) gives the same bits as one that uses them."""
+    sc = scenes.two_spheres(per_sphere=3000, spacing=0.04, steps=5)
+    sc.params = sc.params.replace(box_hi=(5.0, 3.0, 3.0))
+    outs = []
+    for ieee in (False, True):
+        if ieee:
+            os.environ["PBF_IEEE_MATH"] = "1"
+        try:
+            with ctx_for(sc.params, sc.x, sc.v) as c:
+                flags = c.math_flags()
+                c.time_step(nsteps=10)
+                outs.append(c.read())
+        finally:
+            os.environ.pop("PBF_IEEE_MATH", None)
+        if ieee:
+            assert flags == {"div": False, "rcp": False, "sqrt": False}
+        else:
+            assert flags == {"div": True, "rcp": True, "sqrt": True}, flags
+    for a, b in zip(*outs):
+        assert bits_equal(a, b)
+    xg, vg, _ = orc.step(sc.params, sc.x, sc.v, 10, orc.NBR_GRID)
+    assert bits_equal(outs[0][0], xg) and bits_equal(outs[0][1], vg)
+
+
 def test_deterministic_across_runs(wall_state):
     p, x, v = wall_state
     outs = []

@@ -160,6 +160,46 @@ def test_isolated_particle(orc):
     assert np.all(out == xs)
 
 
+def test_lambda_denominator_is_sum_of_constraint_gradients(orc):
+    """lambda_i = -C_i / (sum_k |grad_{p_k} C_i|^2 + eps) with
+    grad_{p_i} C_i = (1/rho0) sum_j g_ij and grad_{p_j} C_i = -(1/rho0) g_ij
+    (Macklin & Mueller 2013, eqs. 8-11; src/particles.cpp:185-204).  Three
+    particles; a dropped |G|^2 term or a per-pair sign slip changes D."""
+    p = dyadic_params()
+    rho0 = p.rest_density
+    x = np.array([[0.0, 0.0, 0.0], [0.05, 0.01, -0.02], [-0.03, 0.04, 0.02]])
+    off, lst = orc.neighbors(p, x, orc.NBR_BRUTE, np.float64)
+    rho, lam = orc.lambda_pass(p, x, off, lst, np.float64)
+    g01 = orc.spiky(p, x[0] - x[1], np.float64)
+    g02 = orc.spiky(p, x[0] - x[2], np.float64)
+    grads = [(g01 + g02) / rho0, -g01 / rho0, -g02 / rho0]  # grad_{p_k} C_0, k = 0, 1, 2
+    D = sum(float(np.dot(gk, gk)) for gk in grads)
+    W = orc.poly6(p, float(np.sum((x[0] - x[1]) ** 2)), np.float64) + \
+        orc.poly6(p, float(np.sum((x[0] - x[2]) ** 2)), np.float64)
+    assert rho[0] == pytest.approx(W, rel=1e-14)
+    assert lam[0] == pytest.approx(-(W / rho0 - 1.0) / (D + p.epsilon), rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 4])
+def test_tensile_correction_at_reference_distances(orc, n):
+    """s_corr = -k (W(r)/W(dq h))^n: exactly -k at r = dq h, and -k/2^n where
+    W(r) = W(dq h)/2, i.e. r^2 = h^2 - 2^(-1/3)(h^2 - (dq h)^2).  Two
+    particles, lambda and gradient from the pinned passes."""
+    h = 0.125
+    k = 0.01
+    p = dyadic_params(h, k_tensile=k, n_tensile=n)
+    dq = p.dq * h
+    for r, ratio in [(dq, 1.0), (math.sqrt(h * h - 2 ** (-1 / 3) * (h * h - dq * dq)), 0.5)]:
+        x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        off, lst = orc.neighbors(p, x, orc.NBR_BRUTE, np.float64)
+        _, lam = orc.lambda_pass(p, x, off, lst, np.float64)
+        out = orc.deltap_pass(p, x, lam, off, lst, np.float64)
+        g = orc.spiky(p, x[0] - x[1], np.float64)
+        s = -k * ratio ** n
+        assert np.allclose(out[0] - x[0], (lam[0] + lam[1] + s) * g / p.rest_density,
+                           rtol=1e-9, atol=1e-18)
+
+
 def test_deltap_conserves_sum_and_vanishes_on_lattice(orc):
     """Pairwise antisymmetry (lambda_i + lambda_j + s_ij) grad W_ij = -(..)_ji:
     sum_i dp_i = 0 away from walls; on a perfect lattice the interior dp is 0."""
