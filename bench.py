@@ -7,10 +7,16 @@ Default workload: the 4,096,000-particle two-sphere collision (h = 0.02031,
 HBM.  Prints ONE JSON line:
 
 * value      particle-steps/s = N * steps / wall time of the timed steps
-             (pbf_step returns after the device finished);
-* roofline   the dominant pass, from HIP events recorded on the context
-             stream: algorithmic bytes per launch / mean launch time,
-             against 8 TB/s; traffic from the committed rocprofv3 counters;
+             (the call returns after the device finished them);
+* roofline   the dominant pass (most time per step), from HIP events recorded
+             around every launch on the context stream during the timed
+             steps: algorithmic state bytes per launch (B/particle table
+             below x N) / mean launch time, against 8 TB/s.  `traffic` is the
+             HBM bytes of one launch from the committed rocprofv3 counters
+             (calibrated FETCH_SIZE + WRITE_SIZE); `valu` is the binding roof
+             of these passes: wave-level VALU instructions of one launch
+             (committed counters) x 64 lanes / mean launch time, against
+             256 CU x 64 lanes x 2.4 GHz;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
@@ -29,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # unpacked FP32/INT32 lane operations per second
 
 # Algorithmic FP32 state bytes per particle of every pass (reads + writes of
 # the particle state the pass needs; neighbour gathers assumed on-chip).
@@ -49,10 +56,11 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_traffic(workload: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), committed as profiles/<workload>_traffic.json."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+def load_counters(workload: str):
+    """Per-pass HBM bytes and VALU instructions of one launch, measured with
+    rocprofv3 --pmc and committed as profiles/<workload>_counters.json
+    (written by fluid_amd.rocprof_summary)."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_counters.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -175,8 +183,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -210,17 +217,16 @@ def main() -> int:
     ctx = Particles(sc.params)
     ctx.load(sc.x, sc.v)
     ctx.time_step(nsteps=args.warmup)
+    ctx.sync()
     log(f"warmup {args.warmup} steps done")
     t0 = time.perf_counter()
-    ctx.time_step(nsteps=args.steps)
+    prof = ctx.time_step_profiled(nsteps=args.steps)  # returns after the device finished
     wall = time.perf_counter() - t0
     value = sc.n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.3e} particle-steps/s")
-
-    prof = ctx.time_step_profiled(nsteps=args.profile_steps)
     nbar, nmax = ctx.neighbor_stats()
     ctx.close()
-    ps = args.profile_steps
+    ps = args.steps
     ncell = 1
     for a in range(3):
         ncell *= int((sc.params.box_hi[a] - sc.params.box_lo[a]) / (sc.params.h * sc.params.cell_scale)) + 1
@@ -241,7 +247,8 @@ def main() -> int:
     dom = max(passes, key=lambda k: passes[k]["ms_per_step"])
     d = passes[dom]
     per_launch_bytes = STATE_BYTES[dom] * sc.n
-    traffic = load_traffic(sc.name)
+    counters = (load_counters(sc.name) or {}).get("passes", {})
+    cp = counters.get(dom, {})
     roof = {
         "bound": "hbm",
         "kernel": dom,
@@ -249,11 +256,27 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": d["state_GBs"] / HBM_PEAK_GBS,
-        "traffic": (traffic or {}).get(dom),
+        "traffic": cp.get("hbm_bytes"),
         "bytes_per_launch": per_launch_bytes,
         "basis": f"state-only {STATE_BYTES[dom]} B/particle x {sc.n} particles per launch",
         "state_list_frac": d["state_list_GBs"] / HBM_PEAK_GBS,
+        "us_per_launch": d["us_per_launch"],
     }
+    if cp.get("valu_insts"):
+        lane_ops = cp["valu_insts"] * 64 / (d["us_per_launch"] * 1e-6)
+        roof["valu"] = {
+            "achieved": lane_ops,
+            "peak": VALU_PEAK_LANE_OPS,
+            "unit": "lane-ops/s",
+            "frac": lane_ops / VALU_PEAK_LANE_OPS,
+            "valu_insts_per_launch": cp["valu_insts"],
+        }
+    for name, pd in passes.items():
+        c = counters.get(name, {})
+        if c.get("hbm_bytes"):
+            pd["hbm_GBs"] = c["hbm_bytes"] / (pd["us_per_launch"] * 1e-6) / 1e9
+        if c.get("valu_insts"):
+            pd["valu_frac"] = c["valu_insts"] * 64 / (pd["us_per_launch"] * 1e-6) / VALU_PEAK_LANE_OPS
     out = {
         "metric": "PBF particle-steps/s @4M",
         "value": value,

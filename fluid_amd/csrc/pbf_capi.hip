@@ -84,7 +84,7 @@ void dfree(T*& p) {
 void free_device(Ctx* x) {
   pbf::DevState& d = x->d;
   dfree(d.posA); dfree(d.posB); dfree(d.velA); dfree(d.velB);
-  dfree(d.predP); dfree(d.predQ); dfree(d.vort); dfree(d.vxsph);
+  dfree(d.predP); dfree(d.predQ); dfree(d.vort); dfree(d.vxsph); dfree(d.xv);
   dfree(d.cell); dfree(d.arrival); dfree(d.cellSorted); dfree(d.tmpId);
   dfree(d.cellCount); dfree(d.cellStart); dfree(d.blockSums);
   dfree(d.nbr); dfree(d.ncnt); dfree(d.flags);
@@ -117,7 +117,7 @@ int ensure_device(Ctx* x) {
 #define DA(ptr, cnt) \
   if (e == hipSuccess) e = dalloc(ptr, cnt)
   DA(d.posA, np); DA(d.posB, np); DA(d.velA, np); DA(d.velB, np);
-  DA(d.predP, np); DA(d.predQ, np); DA(d.vort, np); DA(d.vxsph, np);
+  DA(d.predP, np + 64); DA(d.predQ, np + 64); DA(d.vort, np); DA(d.vxsph, np); DA(d.xv, 2 * np);
   DA(d.cell, np); DA(d.arrival, np); DA(d.cellSorted, np); DA(d.tmpId, np);
   DA(d.cellCount, C + 4); DA(d.cellStart, C + 4); DA(d.blockSums, nb);
   DA(d.nbr, np * K); DA(d.ncnt, np); DA(d.flags, 64);
@@ -659,22 +659,23 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
     if (pass_launches) pass_launches[i] = 0;
   }
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
+  // Events of every launch of every step, read after one synchronisation at
+  // the end, so the steps run back to back as in pbf_step.
   PassEvents pe;
   for (int s = 0; s < nsteps && rc == PBF_OK; s++) {
-    pe.used = 0;
     rc = run_step(x, k, 0, &pe);
-    if (rc == PBF_OK) {
-      hipError_t e = hipStreamSynchronize(x->stream);
-      if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
-    }
-    for (int l = 0; l < pe.used && rc == PBF_OK; l++) {
-      float ms = 0.0f;
-      hipError_t e = hipEventElapsedTime(&ms, pe.ev[(size_t)l * 2], pe.ev[(size_t)l * 2 + 1]);
-      if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
-      pass_ms[pe.pass[(size_t)l]] += ms;
-      if (pass_launches) pass_launches[pe.pass[(size_t)l]] += 1;
-    }
     if (rc == PBF_OK) x->sim_time += (double)dt;
+  }
+  if (rc == PBF_OK) {
+    hipError_t e = hipStreamSynchronize(x->stream);
+    if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
+  }
+  for (int l = 0; l < pe.used && rc == PBF_OK; l++) {
+    float ms = 0.0f;
+    hipError_t e = hipEventElapsedTime(&ms, pe.ev[(size_t)l * 2], pe.ev[(size_t)l * 2 + 1]);
+    if (e != hipSuccess) rc = fail(x, PBF_ERR_HIP, hipGetErrorString(e));
+    pass_ms[pe.pass[(size_t)l]] += ms;
+    if (pass_launches) pass_launches[pe.pass[(size_t)l]] += 1;
   }
   for (hipEvent_t e : pe.ev) (void)hipEventDestroy(e);
   if (rc != PBF_OK) return rc;

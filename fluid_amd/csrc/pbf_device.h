@@ -283,6 +283,14 @@ __device__ __forceinline__ float3 spiky_sel(const SimConst& c, float rx, float r
   return make_float3(rc * (coef * rx), rc * (coef * ry), rc * (coef * rz));
 }
 
+// Element j of a particle array, addressed with a 32-bit byte offset from
+// the array base (arrays hold fewer than 2^28 float4), which lets the
+// compiler use the scalar-base + 32-bit-offset form of the global loads.
+template <class T>
+__device__ __forceinline__ const T& at(const T* __restrict__ p, uint32_t j) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + j * (uint32_t)sizeof(T));
+}
+
 // Visits the neighbours of particle i in list order.  Groups of U
 // independent index loads, then U independent gathers, are issued before the
 // U bodies run, so the memory latency of a group overlaps; body(j, data) is

@@ -33,6 +33,7 @@ struct DevState {
   float4* predQ = nullptr;  // x* pong
   float4* vort = nullptr;   // omega (xyz)
   float4* vxsph = nullptr;  // v after XSPH (xyz), density (w)
+  float4* xv = nullptr;     // (x*, v) pairs after the solve, 2 per particle
   uint32_t* cell = nullptr;       // cell of particle, unsorted
   uint32_t* arrival = nullptr;    // arrival rank within its cell
   uint32_t* cellSorted = nullptr; // cell of particle, sorted

@@ -34,12 +34,17 @@ constexpr int kBlock = 256;
 #ifndef PBF_UNROLL_XSPH
 #define PBF_UNROLL_XSPH 8
 #endif
+// XSPH gathers x* and v of a neighbour from one 32-byte record written by
+// the last delta-p pass (one cache line per neighbour instead of two).
+#ifndef PBF_XV
+#define PBF_XV 1
+#endif
 constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
 constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers two float4 per neighbour
 
 // Device pointers and sizes of one context, passed by value to every kernel.
 struct Dev {
-  float4 *posA, *posB, *velA, *velB, *predP, *predQ, *vort, *vxsph;
+  float4 *posA, *posB, *velA, *velB, *predP, *predQ, *vort, *vxsph, *xv;
   uint32_t *cell, *arrival, *cellSorted, *tmpId, *cellCount, *cellStart, *nbr, *ncnt, *flags;
   const SceneGrid* grids;
   const uint32_t* slotScene;
@@ -51,7 +56,7 @@ Dev dev_of(const Ctx& x) {
   const DevState& s = x.d;
   Dev d;
   d.posA = s.posA; d.posB = s.posB; d.velA = s.velA; d.velB = s.velB;
-  d.predP = s.predP; d.predQ = s.predQ; d.vort = s.vort; d.vxsph = s.vxsph;
+  d.predP = s.predP; d.predQ = s.predQ; d.vort = s.vort; d.vxsph = s.vxsph; d.xv = s.xv;
   d.cell = s.cell; d.arrival = s.arrival; d.cellSorted = s.cellSorted; d.tmpId = s.tmpId;
   d.cellCount = s.cellCount; d.cellStart = s.cellStart; d.nbr = s.nbr; d.ncnt = s.ncnt;
   d.flags = s.flags; d.grids = s.grids; d.slotScene = s.slotScene;
@@ -307,19 +312,19 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
     cnt++;
   };
 #else
-  uint32_t buf[kChunk];
-#pragma unroll
-  for (int u = 0; u < kChunk; u++) buf[u] = 0u;
+  // A 4-entry shift register: the latest hit enters at b3, and after every
+  // fourth hit b0..b3 hold the chunk in order.
+  uint32_t b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u;
   auto push = [&](uint32_t j) {
-    const uint32_t slot = cnt & 3u;
-#pragma unroll
-    for (int u = 0; u < kChunk; u++) buf[u] = (slot == (uint32_t)u) ? j : buf[u];
+    b0 = b1;
+    b1 = b2;
+    b2 = b3;
+    b3 = j;
     cnt++;
     if ((cnt & 3u) == 0u) {
       const int chunk = (int)(cnt >> 2) - 1;
       if (chunk < d.K4)
-        *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
-            make_uint4(buf[0], buf[1], buf[2], buf[3]);
+        *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) = make_uint4(b0, b1, b2, b3);
     }
   };
 #endif
@@ -347,18 +352,17 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
       const uint32_t s = d.cellStart[row + (uint32_t)x0];
       const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
       for (uint32_t jb = s; jb < e; jb += 32) {
-        const uint32_t je = min(e, jb + 32u), last = je - 1u;
+        const uint32_t len = min(e - jb, 32u);
         uint32_t mask = 0;
-        for (uint32_t j = jb; j < je; j += 4) {
-          // Loads past the range repeat the last candidate; their bits are
-          // masked off.
-          const float4 a = pred[j], b = pred[min(j + 1u, last)], c4 = pred[min(j + 2u, last)],
-                       q = pred[min(j + 3u, last)];
-          const uint32_t m4 = hit(a) | (hit(b) << 1) | (hit(c4) << 2) | (hit(q) << 3);
-          const uint32_t left = je - j;
-          const uint32_t valid = left >= 4u ? 0xFu : ((1u << left) - 1u);
-          mask |= (m4 & valid) << (j - jb);
+        // Groups of 4 may read up to 3 candidates past the range (the x*
+        // arrays are padded by 64 entries); their bits are masked off.
+        for (uint32_t o = 0; o < len; o += 4) {
+          const uint32_t j = jb + o;
+          const float4 a = at(pred, j), b = at(pred, j + 1u), c4 = at(pred, j + 2u),
+                       q = at(pred, j + 3u);
+          mask |= (hit(a) | (hit(b) << 1) | (hit(c4) << 2) | (hit(q) << 3)) << o;
         }
+        if (len < 32u) mask &= (1u << len) - 1u;
         const uint32_t self = i - jb;
         if (self < 32u) mask &= ~(1u << self);
         while (mask) {
@@ -370,11 +374,17 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
     }
   }
 #if !PBF_NBR_PUSH
-  if (cnt & 3u) {
+  // Partial last chunk: its r = cnt % 4 entries are b(4-r)..b3.  The slots
+  // past them are filled with a repeat of the first, so every slot of the
+  // chunks in use holds a particle index.
+  const uint32_t r = cnt & 3u;
+  if (r) {
     const int chunk = (int)(cnt >> 2);
+    const uint32_t e0 = r == 1u ? b3 : (r == 2u ? b2 : b1);
+    const uint32_t e1 = r == 1u ? b3 : (r == 2u ? b3 : b2);
+    const uint32_t e2 = r == 3u ? b3 : e0;
     if (chunk < d.K4)
-      *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) =
-          make_uint4(buf[0], buf[1], buf[2], buf[3]);
+      *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) = make_uint4(e0, e1, e2, e0);
   }
 #endif
   if (cnt > cap) {
@@ -392,7 +402,7 @@ __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimCo
   const int cnt = (int)d.ncnt[i];
   float dens = 0.0f, denom = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
   visit_neighbors<kUnroll>(
-      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -424,7 +434,7 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   const int cnt = (int)d.ncnt[i];
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
   visit_neighbors<kUnroll>(
-      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
@@ -443,8 +453,13 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   out[i] = make_float4(p.x, p.y, p.z, 0.0f);
   if (LAST) {
     const float4 x = d.posB[i];
-    d.velB[i] = make_float4(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y),
-                            c.inv_dt * (p.z - x.z), 0.0f);
+    const float4 v = make_float4(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y),
+                                 c.inv_dt * (p.z - x.z), 0.0f);
+    d.velB[i] = v;
+#if PBF_XV
+    d.xv[2 * (size_t)i] = make_float4(p.x, p.y, p.z, 0.0f);
+    d.xv[2 * (size_t)i + 1] = v;
+#endif
   }
 }
 
@@ -455,6 +470,7 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
                                          const float4* __restrict__ in,
                                          float4* __restrict__ out) {
   const float4* __restrict__ vel = d.velB;
+  const float4* __restrict__ xv = d.xv;  // (x*, v) of particle j at 2j, 2j+1
   const float4 xi = in[i];
   const float4 vi = vel[i];
   const int cnt = (int)d.ncnt[i];
@@ -462,8 +478,15 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
   struct XV {
     float4 x, v;
   };
-  visit_neighbors<kUnrollXsph>(
-      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{in[j], vel[j]}; },
+  for_each_neighbor<kUnrollXsph>(
+      i, cnt, d.K4, d.nbr,
+      [&](uint32_t j) {
+#if PBF_XV
+        return XV{at(xv, 2 * j), at(xv, 2 * j + 1)};
+#else
+        return XV{at(in, j), at(vel, j)};
+#endif
+      },
       [&](uint32_t, const XV& q) {
         const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
         const float rx = xi.x - q.x.x, ry = xi.y - q.x.y, rz = xi.z - q.x.z;
@@ -492,7 +515,7 @@ __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimCons
   const int cnt = (int)d.ncnt[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   visit_neighbors<kUnroll>(
-      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return in[j]; },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;

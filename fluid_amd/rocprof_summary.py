@@ -1,0 +1,184 @@
+"""Summaries of rocprofv3 output for the profiles/ folder.
+
+    python -m fluid_amd.rocprof_summary stats DIR --command "..." --out FILE.txt
+    python -m fluid_amd.rocprof_summary counters --fetch DIR --write DIR --sq DIR
+        --workload NAME --command "..." --out-json FILE.json --out-txt FILE.txt
+
+`stats` turns run_kernel_stats.csv (from --kernel-trace --stats) into a
+table.  `counters` reads three --pmc runs of the same program: FETCH_SIZE,
+WRITE_SIZE and an SQ pass (SQ_INSTS_VALU, SQ_WAVES, ... with --kernel-trace).
+FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by the factor that
+makes the calibration copy (the 256 MiB device-to-device copy that
+`bench.py --calibrate` runs first, a __amd_rocclr_copyBuffer dispatch) read
+and write 256 MiB.  The JSON holds, per pass of the step, the calibrated HBM
+bytes and the wave-level VALU instructions of one launch; bench.py reads it.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import os
+import re
+
+# Kernel name prefix -> pass name of the step (pbf.h PBF_PASS_* order).
+PASS_OF = [
+    ("k_predict", "predict"),
+    ("k_scan_block", "scan"),
+    ("k_scan_partials", "scan"),
+    ("k_scan_add", "scan"),
+    ("k_scatter", "sort"),
+    ("k_rank", "sort"),
+    ("k_nbr_build", "neighbors"),
+    ("k_lambda", "lambda"),
+    ("k_deltap", "deltap"),
+    ("k_xsph", "xsph"),
+    ("k_vort_commit", "vorticity"),
+]
+COPY_BYTES = 256 << 20
+
+
+def short(name: str) -> str:
+    name = name.replace("pbf::(anonymous namespace)::", "").replace("void ", "")
+    return re.sub(r"\(.*", "", name)
+
+
+def pass_of(kernel: str):
+    for prefix, p in PASS_OF:
+        if kernel.startswith(prefix):
+            return p
+    return None
+
+
+def read_csv(path: str):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def stats(args) -> None:
+    rows = read_csv(os.path.join(args.dir, "run_kernel_stats.csv"))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = [f"# rocprofv3 --kernel-trace --stats: {args.title}", "",
+             f"command: {args.command}", "",
+             f"{'kernel':40s} {'calls':>6s} {'avg us':>9s} {'min us':>9s} {'max us':>9s} {'% time':>7s}"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"{short(r['Name'])[:40]:40s} {int(r['Calls']):6d} "
+                     f"{float(r['AverageNs']) / 1e3:9.1f} {float(r['MinNs']) / 1e3:9.1f} "
+                     f"{float(r['MaxNs']) / 1e3:9.1f} {100 * float(r['TotalDurationNs']) / total:7.2f}")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def per_kernel(path: str):
+    """Mean of every counter per (kernel, grid size), over dispatches, and
+    the number of dispatches ("dispatches")."""
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in read_csv(os.path.join(path, "run_counter_collection.csv")):
+        k = short(r["Kernel_Name"])
+        vals[(k, r["Grid_Size"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    out = {}
+    for key, d in vals.items():
+        out[key] = {c: sum(v) / len(v) for c, v in d.items()}
+        out[key]["dispatches"] = max(len(v) for v in d.values())
+    return out
+
+
+def durations(path: str):
+    f = os.path.join(path, "run_kernel_trace.csv")
+    if not os.path.exists(f):
+        return {}
+    acc = collections.defaultdict(list)
+    for r in read_csv(f):
+        k = short(r["Kernel_Name"])
+        acc[(k, r["Grid_Size_X"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    return {key: sum(v) / len(v) for key, v in acc.items()}
+
+
+def calibration(table, counter: str) -> float:
+    """HBM bytes per counted KiB, from the largest copyBuffer dispatch."""
+    best = max((v[counter] for (k, _), v in table.items() if k.startswith("__amd_rocclr_copyBuffer")),
+               default=0.0)
+    if best <= 0:
+        raise SystemExit(f"no calibration copy found for {counter}: run bench.py with --calibrate")
+    return COPY_BYTES / best
+
+
+def counters(args) -> None:
+    fetch, write, sq = per_kernel(args.fetch), per_kernel(args.write), per_kernel(args.sq)
+    dur = durations(args.sq)
+    ff, wf = calibration(fetch, "FETCH_SIZE"), calibration(write, "WRITE_SIZE")
+    out = {"workload": args.workload, "fetch_factor": ff, "write_factor": wf, "passes": {},
+           "kernels": {}}
+    lines = [f"# rocprofv3 --pmc summary: {args.workload}", "",
+             f"command (one run per counter set): {args.command}", "",
+             f"HBM bytes = FETCH_SIZE KiB x 1024 x {ff / 1024:.4f} + WRITE_SIZE KiB x 1024 x {wf / 1024:.4f}: "
+             f"the 256 MiB calibration copy counts {COPY_BYTES / ff:.0f} KiB FETCH_SIZE and "
+             f"{COPY_BYTES / wf:.0f} KiB WRITE_SIZE",
+             "", f"{'kernel':28s} {'grid':>8s} {'us':>7s} {'HBM rd MB':>10s} {'HBM wr MB':>10s} "
+             f"{'VALU Minst':>10s} {'waves':>7s} {'VMEM rd M':>9s} {'LDS bank cf':>11s}"]
+    pass_recs = collections.defaultdict(list)
+    for key in sorted(sq, key=lambda k: -sq[k].get("SQ_INSTS_VALU", 0)):
+        k, grid = key
+        p = pass_of(k)
+        if p is None:
+            continue
+        rd = fetch.get(key, {}).get("FETCH_SIZE", 0.0) * ff
+        wr = write.get(key, {}).get("WRITE_SIZE", 0.0) * wf
+        s = sq[key]
+        rec = {"pass": p, "grid": int(grid), "us": dur.get(key), "hbm_read_bytes": rd,
+               "hbm_write_bytes": wr, "valu_insts": s.get("SQ_INSTS_VALU", 0.0),
+               "waves": s.get("SQ_WAVES", 0.0), "vmem_rd_insts": s.get("SQ_INSTS_VMEM_RD", 0.0),
+               "lds_bank_conflict": s.get("SQ_LDS_BANK_CONFLICT", 0.0),
+               "lds_idx_active": s.get("SQ_LDS_IDX_ACTIVE", 0.0)}
+        out["kernels"][f"{k}[{grid}]"] = rec
+        us = f"{rec['us']:7.1f}" if rec["us"] else "      -"
+        lines.append(f"{k[:28]:28s} {grid:>8s} {us} {rd / 1e6:10.1f} {wr / 1e6:10.1f} "
+                     f"{rec['valu_insts'] / 1e6:10.1f} {rec['waves']:7.0f} "
+                     f"{rec['vmem_rd_insts'] / 1e6:9.2f} {rec['lds_bank_conflict']:11.0f}")
+        pass_recs[p].append((k, s["dispatches"], rec))
+    for p, recs in pass_recs.items():
+        # Kernels of one pass that run one after the other (scan, sort) add
+        # up; the template variants of one kernel (the last delta-p of a
+        # step) are averaged, weighted by their dispatches.
+        fam = collections.defaultdict(list)
+        for k, n, rec in recs:
+            fam[re.sub(r"<.*", "", k)].append((n, rec))
+        tot = {}
+        for f_ in ("hbm_read_bytes", "hbm_write_bytes", "valu_insts"):
+            tot[f_] = sum(sum(n * r[f_] for n, r in v) / sum(n for n, _ in v) for v in fam.values())
+        tot["hbm_bytes"] = tot["hbm_read_bytes"] + tot["hbm_write_bytes"]
+        out["passes"][p] = tot
+    lines += ["", "per launch of each pass (HBM bytes = calibrated FETCH + WRITE):"]
+    for p, t in out["passes"].items():
+        lines.append(f"  {p:10s} HBM {t['hbm_bytes'] / 1e6:8.1f} MB   VALU {t['valu_insts'] / 1e6:7.1f} M wave-instructions")
+    with open(args.out_json, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    with open(args.out_txt, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("dir")
+    a.add_argument("--command", required=True)
+    a.add_argument("--title", default="")
+    a.add_argument("--out", required=True)
+    b = sub.add_parser("counters")
+    b.add_argument("--fetch", required=True)
+    b.add_argument("--write", required=True)
+    b.add_argument("--sq", required=True)
+    b.add_argument("--workload", required=True)
+    b.add_argument("--command", required=True)
+    b.add_argument("--out-json", required=True)
+    b.add_argument("--out-txt", required=True)
+    args = ap.parse_args()
+    stats(args) if args.cmd == "stats" else counters(args)
+
+
+if __name__ == "__main__":
+    main()
