@@ -141,6 +141,39 @@ def batched_bench(steps: int, looped_steps: int):
     }
 
 
+def density_field_bench(ctx, sc):
+    """Density on the marching-cubes corner lattice of the scene's box at
+    step h/2 (the reference's H * FSTEPSIZE_RATIO, src/particles.cpp:396-397),
+    after the timed steps: the first call sorts the committed state by cell,
+    the second reuses it."""
+    import numpy as np
+
+    p = sc.params
+    lo = np.array(p.box_lo, np.float32)
+    hi = np.array(p.box_hi, np.float32)
+    step = np.float32(p.h * 0.5)
+    counts = [int((hi[a] - lo[a]) / step) + 2 for a in range(3)]
+    m = int(np.prod(counts))
+    t0 = time.perf_counter()
+    ctx.density_lattice(lo, hi, float(step), counts)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lat = ctx.density_lattice(lo, hi, float(step), counts)
+    again = time.perf_counter() - t0
+    log(f"density lattice {counts} = {m} points: {first * 1e3:.1f} ms (with sort), "
+        f"{again * 1e3:.1f} ms cached")
+    return {
+        "metric": "density lattice points/s",
+        "points": m,
+        "counts": counts,
+        "step": float(step),
+        "ms_with_sort": first * 1e3,
+        "ms": again * 1e3,
+        "points_per_s": m / again,
+        "surface_points": int((lat >= 0.95 * p.rest_density).sum()),
+    }
+
+
 def adjoint_bench():
     """32^3 block, 60 steps, checkpoints every 10: wall time of the backward
     pass (recompute included) per step, and of the forward pass."""
@@ -225,6 +258,7 @@ def main() -> int:
     value = sc.n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.3e} particle-steps/s")
     nbar, nmax = ctx.neighbor_stats()
+    field = density_field_bench(ctx, sc)
     ctx.close()
     ps = args.steps
     ncell = 1
@@ -302,6 +336,7 @@ def main() -> int:
         },
         "roofline": roof,
         "passes": passes,
+        "density_field": field,
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_steps, args.looped_steps)

@@ -55,6 +55,11 @@ SIGNATURES = {
     "pbf_step": (C.c_int, [_P, C.c_int32]),
     "pbf_read": (C.c_int, [_P, _F, _F, _F]),
     "pbf_estimate_densities": (C.c_int, [_P, _F]),
+    "pbf_density_at": (C.c_int, [_P, C.c_int32, _F, C.c_int64, _F]),
+    "pbf_save_snapshot": (C.c_int, [_P, C.c_char_p]),
+    "pbf_load_snapshot": (C.c_int, [_P, C.c_char_p]),
+    "pbf_density_lattice": (C.c_int, [_P, C.c_int32, _F, _F, C.c_float,
+                                      C.POINTER(C.c_int32), _F]),
     "pbf_simulate_time": (C.c_double, [_P]),
     "pbf_count": (C.c_int64, [_P]),
     "pbf_last_error": (C.c_char_p, [_P]),

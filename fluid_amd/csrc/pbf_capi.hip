@@ -469,9 +469,19 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   return PBF_OK;
 }
 
+namespace {
+int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_t n);
+}
+
 PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  return load_impl(x, xyz, v, nullptr, n);
+}
+
+namespace {
+// pbf_load, with the last densities (rho, may be NULL) restored as well.
+int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_t n) {
   if (!xyz || n < 1) return fail(x, PBF_ERR_INVALID, "xyz must be non-NULL and n >= 1");
   if (n >= (int64_t)1 << 31) return fail(x, PBF_ERR_INVALID, "more than 2^31 particles");
   if (x->nscenes == 1) {
@@ -511,8 +521,9 @@ PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
     float idf;
     memcpy(&idf, &id, 4);
     hp[(size_t)i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
-    hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0f)
-                      : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float r = rho ? rho[i] : 0.0f;
+    hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], r)
+                      : make_float4(0.0f, 0.0f, 0.0f, r);
   }
   pbf::DevState& d = x->d;
   HIPCHK(x, hipMemcpyAsync(d.posA, hp.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
@@ -543,8 +554,10 @@ PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
   x->loaded = true;
   x->debug_dirty = false;
   x->has_nbrs = false;
+  x->field_ready = false;
   return PBF_OK;
 }
+}  // namespace
 
 namespace {
 
@@ -565,6 +578,7 @@ int sync_check(Ctx* x) {
 // Enqueues nsteps steps with the context's schedule.
 int enqueue_steps(Ctx* x, float dt, int nsteps) {
   if (nsteps == 0) return PBF_OK;
+  x->field_ready = false;
   const SimConst k = pbf::consts_for(*x, dt);
   if (x->schedule == PBF_SCHED_PERSISTENT) {
     const int per = pbf::persistent_max_blocks_per_cu(x->pblock);
@@ -658,6 +672,7 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
     pass_ms[i] = 0.0f;
     if (pass_launches) pass_launches[i] = 0;
   }
+  x->field_ready = false;
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
   // Events of every launch of every step, read after one synchronisation at
   // the end, so the steps run back to back as in pbf_step.
@@ -696,6 +711,7 @@ PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
   if (!is_finite_f(dt) || !(dt > 0.0f) || stage < PBF_STAGE_PREDICT || stage > PBF_STAGE_XSPH)
     return fail(x, PBF_ERR_INVALID, "dt must be > 0 and stage a PBF_STAGE_* value");
   const SimConst k = pbf::consts_for(*x, dt);
+  x->field_ready = false;
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
   drop_graph(x);
   x->debug_dirty = true;
@@ -726,6 +742,142 @@ PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
   return PBF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Snapshot file layout (little-endian), see pbf_save_snapshot in pbf.h.
+constexpr char kSnapMagic[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '1'};
+struct SnapHeader {
+  char magic[8];
+  uint32_t header_bytes;  // sizeof(SnapHeader) = 64
+  uint32_t params_bytes;  // sizeof(pbf_params)
+  int64_t n;
+  int32_t nscenes;
+  int32_t reserved0;
+  double sim_time;
+  uint64_t checksum;      // FNV-1a 64 of every byte after the header
+  uint64_t reserved1[2];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
+
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+  }
+};
+
+// The payload after the header: params, per-scene counts and boxes, x, v, rho.
+struct SnapBody {
+  pbf_params p;
+  std::vector<int64_t> counts;
+  std::vector<float> boxes;  // lo[3], hi[3] per scene
+  std::vector<float> x, v, rho;
+  template <class F>
+  void each(F f) {
+    f(&p, sizeof p);
+    f(counts.data(), counts.size() * sizeof(int64_t));
+    f(boxes.data(), boxes.size() * sizeof(float));
+    f(x.data(), x.size() * sizeof(float));
+    f(v.data(), v.size() * sizeof(float));
+    f(rho.data(), rho.size() * sizeof(float));
+  }
+};
+
+void scene_layout(const Ctx* x, SnapBody* b) {
+  b->p = x->p;
+  b->counts = x->counts;
+  b->boxes.clear();
+  for (const SceneGrid& g : x->grids) {
+    b->boxes.insert(b->boxes.end(), g.lo, g.lo + 3);
+    b->boxes.insert(b->boxes.end(), g.hi, g.hi + 3);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+PBF_API int pbf_save_snapshot(pbf_ctx* c, const char* path) {
+  Ctx* x = C(c);
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (!path) return fail(x, PBF_ERR_INVALID, "path is NULL");
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  SnapBody b;
+  scene_layout(x, &b);
+  const size_t n = (size_t)x->n;
+  b.x.resize(3 * n);
+  b.v.resize(3 * n);
+  b.rho.resize(n);
+  if ((rc = pbf_read(c, b.x.data(), b.v.data(), b.rho.data())) != PBF_OK) return rc;
+  SnapHeader hd{};
+  memcpy(hd.magic, kSnapMagic, 8);
+  hd.header_bytes = sizeof(SnapHeader);
+  hd.params_bytes = sizeof(pbf_params);
+  hd.n = x->n;
+  hd.nscenes = x->nscenes;
+  hd.sim_time = x->sim_time;
+  Fnv f;
+  b.each([&](const void* p, size_t k) { f.add(p, k); });
+  hd.checksum = f.h;
+  FILE* fp = fopen(path, "wb");
+  if (!fp) return fail(x, PBF_ERR_INVALID, std::string("cannot open ") + path + " for writing");
+  bool ok = fwrite(&hd, sizeof hd, 1, fp) == 1;
+  b.each([&](const void* p, size_t k) { ok = ok && (k == 0 || fwrite(p, 1, k, fp) == k); });
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) return fail(x, PBF_ERR_INVALID, std::string("write to ") + path + " failed");
+  return PBF_OK;
+}
+
+PBF_API int pbf_load_snapshot(pbf_ctx* c, const char* path) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (!path) return fail(x, PBF_ERR_INVALID, "path is NULL");
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return fail(x, PBF_ERR_INVALID, std::string("cannot open ") + path);
+  SnapHeader hd{};
+  const bool head_ok = fread(&hd, sizeof hd, 1, fp) == 1;
+  std::string why;
+  if (!head_ok || memcmp(hd.magic, kSnapMagic, 8) != 0 || hd.header_bytes != sizeof(SnapHeader))
+    why = "not a snapshot file";
+  else if (hd.params_bytes != sizeof(pbf_params))
+    why = "snapshot written with a different pbf_params layout";
+  else if (hd.nscenes != x->nscenes || hd.n < 1 || hd.n >= ((int64_t)1 << 31))
+    why = "snapshot holds " + std::to_string(hd.nscenes) + " scene(s), the context " +
+          std::to_string(x->nscenes);
+  SnapBody b, mine;
+  if (why.empty()) {
+    const size_t n = (size_t)hd.n;
+    b.counts.resize((size_t)hd.nscenes);
+    b.boxes.resize(6 * (size_t)hd.nscenes);
+    b.x.resize(3 * n);
+    b.v.resize(3 * n);
+    b.rho.resize(n);
+    bool ok = true;
+    b.each([&](void* p, size_t k) { ok = ok && (k == 0 || fread(p, 1, k, fp) == k); });
+    Fnv f;
+    b.each([&](const void* p, size_t k) { f.add(p, k); });
+    if (!ok || fgetc(fp) != EOF) why = "snapshot file truncated or too long";
+    else if (f.h != hd.checksum) why = "snapshot checksum mismatch";
+  }
+  fclose(fp);
+  if (why.empty()) {
+    scene_layout(x, &mine);
+    if (memcmp(&b.p, &mine.p, sizeof(pbf_params)) != 0)
+      why = "snapshot written with different parameters";
+    else if (x->nscenes > 1 && b.counts != mine.counts)
+      why = "snapshot scene particle counts differ from the context's";
+    else if (memcmp(b.boxes.data(), mine.boxes.data(), b.boxes.size() * sizeof(float)) != 0)
+      why = "snapshot scene boxes differ from the context's";
+  }
+  if (!why.empty()) return fail(x, PBF_ERR_INVALID, why + ": " + path);
+  const int rc = load_impl(x, b.x.data(), b.v.data(), b.rho.data(), hd.n);
+  if (rc != PBF_OK) return rc;
+  x->sim_time = hd.sim_time;
+  return PBF_OK;
+}
+
 PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
   Ctx* x = C(c);
   int rc = check_ready(x);
@@ -740,7 +892,77 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
   std::swap(x->d.posA, x->d.posB);
   std::swap(x->d.velA, x->d.velB);
   x->has_nbrs = false;
+  x->field_ready = true;
   return PBF_OK;
+}
+
+// Density field of one scene at m query points (pts) or on a lattice.  The
+// committed state is re-sorted by cell first, as pbf_estimate_densities does.
+static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::DensityLattice* L,
+                         int64_t m, float* out) {
+  int rc = check_ready(x);
+  if (rc != PBF_OK) return rc;
+  if (x->debug_dirty) return fail(x, PBF_ERR_STATE, "state left by pbf_step_debug: reload first");
+  if (scene < 0 || scene >= x->nscenes) return fail(x, PBF_ERR_INVALID, "scene out of range");
+  if (m < 0 || m > (int64_t)INT32_MAX) return fail(x, PBF_ERR_INVALID, "0 <= m < 2^31 required");
+  if (!out || (!L && m > 0 && !pts)) return fail(x, PBF_ERR_INVALID, "NULL argument");
+  if (!L)
+    for (int64_t i = 0; i < 3 * m; i++)
+      if (!is_finite_f(pts[i])) return fail(x, PBF_ERR_INVALID, "non-finite query point");
+  if (m == 0) return PBF_OK;
+  const SimConst k = pbf::consts_for(*x, x->p.dt);
+  float *dp = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc((void**)&dout, (size_t)m * sizeof(float));
+  if (e == hipSuccess && !L) e = hipMalloc((void**)&dp, (size_t)m * 3 * sizeof(float));
+  if (e == hipSuccess && !L)
+    e = hipMemcpyAsync(dp, pts, (size_t)m * 3 * sizeof(float), hipMemcpyHostToDevice, x->stream);
+  // Sorted once per committed state: later queries reuse the cell table.
+  const bool sort = !x->field_ready;
+  if (e == hipSuccess && sort) e = pbf::launch_sort_committed(*x, k);
+  if (e == hipSuccess) e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, dout);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout, (size_t)m * sizeof(float), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  (void)hipFree(dp);
+  (void)hipFree(dout);
+  if (e != hipSuccess) {
+    x->field_ready = false;
+    return fail(x, PBF_ERR_HIP, std::string("density field: ") + hipGetErrorString(e));
+  }
+  if (sort) {
+    // The sort left the committed state, re-sorted, in posB/velB.
+    drop_graph(x);
+    std::swap(x->d.posA, x->d.posB);
+    std::swap(x->d.velA, x->d.velB);
+    x->has_nbrs = false;
+    x->field_ready = true;
+  }
+  return PBF_OK;
+}
+
+PBF_API int pbf_density_at(pbf_ctx* c, int32_t scene, const float* pts, int64_t m, float* out) {
+  return density_field(C(c), scene, pts, nullptr, m, out);
+}
+
+PBF_API int pbf_density_lattice(pbf_ctx* c, int32_t scene, const float* lo, const float* hi,
+                                float step, const int32_t* counts, float* out) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (!lo || !hi || !counts) return fail(x, PBF_ERR_INVALID, "NULL argument");
+  pbf::DensityLattice L{};
+  int64_t m = 1;
+  if (!is_finite_f(step) || !(step > 0.0f)) return fail(x, PBF_ERR_INVALID, "step > 0 required");
+  L.step = step;
+  for (int a = 0; a < 3; a++) {
+    if (!is_finite_f(lo[a]) || !is_finite_f(hi[a]) || counts[a] < 1)
+      return fail(x, PBF_ERR_INVALID, "finite lo/hi and counts >= 1 required");
+    L.lo[a] = lo[a];
+    L.hi[a] = hi[a];
+    L.n[a] = (uint32_t)counts[a];
+    m *= counts[a];
+    if (m > (int64_t)INT32_MAX) return fail(x, PBF_ERR_INVALID, "lattice of 2^31 points or more");
+  }
+  return density_field(x, scene, nullptr, &L, m, out);
 }
 
 PBF_API double pbf_simulate_time(const pbf_ctx* c) {

@@ -69,6 +69,7 @@ struct Ctx {
   bool loaded = false;
   bool debug_dirty = false;
   bool has_nbrs = false;
+  bool field_ready = false;          // committed state sorted by cell, cell table current
   const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop
   const float4* dbg_vel = nullptr;   // velocity buffer current at the last debug stop
   int schedule = PBF_SCHED_FLAT;     // PBF_SCHED_*
@@ -108,6 +109,18 @@ hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int fi
 hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, uint32_t* dst);
 hipError_t launch_nbr_export(Ctx& x, uint32_t* counts, uint32_t* ids, int stride);
 hipError_t launch_densities(Ctx& x, const SimConst& c, float* out);
+// Sorts the committed state by cell (x into predQ/posB, v into velB) and
+// builds the cell table, as the load-time density pass does.
+hipError_t launch_sort_committed(Ctx& x, const SimConst& c);
+// Lattice of density query points: (min(hi, lo + i*step), ...), i < n.
+struct DensityLattice {
+  float lo[3], hi[3], step;
+  uint32_t n[3];
+};
+// Density field of scene `scene` at m points (pts, or the lattice when
+// given); needs launch_sort_committed first.
+hipError_t launch_density_field(Ctx& x, const SimConst& c, int scene, const float* pts,
+                                const DensityLattice* lattice, uint32_t m, float* out);
 hipError_t launch_persistent(Ctx& x, const SimConst& c, int nsteps, int block, int grid);
 hipError_t launch_verify_math(float b, float rb, int div_e_lo, int div_e_hi, uint32_t* bad,
                               hipStream_t s);

@@ -541,14 +541,16 @@ __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimCons
   d.velA[i] = v;
 }
 
-// Load-time density (src/particles.cpp:440-444 -> :158-163): every particle
-// within the 27 cells, self included, r = x_j - x_i.  Written by id.
-__device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimConst& c,
-                                            const SceneGrid& g, float* __restrict__ out) {
+// Density of the committed particles at point p (estimateDensityAt,
+// src/particles.cpp:446-453): the particles of the 27 cells around p (the
+// state sorted by cell, x in predQ), in (cell, id) order, r = x_j - p.  A
+// point outside the grid takes the clamped boundary cell, whose
+// neighbourhood still holds every particle within h of it.
+__device__ __forceinline__ float density_point(float3 p, const Dev& d, const SimConst& c,
+                                               const SceneGrid& g) {
   const float4* __restrict__ pred = d.predQ;
-  const float4 xi = pred[i];
   int cc[3];
-  cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
+  cell_lin(g, p, cc);
   const int x0 = cc[0] > 0 ? cc[0] - 1 : 0;
   const int x1 = cc[0] + 1 < g.dims[0] ? cc[0] + 1 : g.dims[0] - 1;
   float dens = 0.0f;
@@ -564,12 +566,20 @@ __device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimC
       const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
       for (uint32_t j = s; j < e; j++) {
         const float4 xj = pred[j];
-        const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+        const float rx = xj.x - p.x, ry = xj.y - p.y, rz = xj.z - p.z;
         dens = dens + poly6(c, rx * rx + ry * ry + rz * rz);
       }
     }
   }
-  out[id_of(d.posB[i])] = dens;
+  return dens;
+}
+
+// Load-time density (src/particles.cpp:440-444 -> :158-163): the density at
+// the particle's own position, self included.  Written by id.
+__device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimConst& c,
+                                            const SceneGrid& g, float* __restrict__ out) {
+  const float4 xi = d.predQ[i];
+  out[id_of(d.posB[i])] = density_point(make_float3(xi.x, xi.y, xi.z), d, c, g);
 }
 
 // ------------------------------------------------------- flat schedule --
@@ -636,6 +646,27 @@ __global__ __launch_bounds__(kBlock) void k_densities(Dev d, SimConst c, SceneGr
                                                       float* out) {
   const uint32_t i = flat_index();
   if (i < d.n) density_one(i, d, c, grid_of<MULTI>(i, g0, d), out);
+}
+
+// Density field at m points of one scene: pts[3q..3q+2], or with LATTICE the
+// point (min(hi, lo + ix*step), ...) of q = (iz*ny + iy)*nx + ix.
+template <bool LATTICE>
+__global__ __launch_bounds__(kBlock) void k_density_field(Dev d, SimConst c, SceneGrid g,
+                                                          const float* __restrict__ pts,
+                                                          DensityLattice L, uint32_t m,
+                                                          float* __restrict__ out) {
+  const uint32_t q = flat_index();
+  if (q >= m) return;
+  float3 p;
+  if (LATTICE) {
+    const uint32_t ix = q % L.n[0], iy = (q / L.n[0]) % L.n[1], iz = q / (L.n[0] * L.n[1]);
+    p = make_float3(smin(L.hi[0], L.lo[0] + (float)ix * L.step),
+                    smin(L.hi[1], L.lo[1] + (float)iy * L.step),
+                    smin(L.hi[2], L.lo[2] + (float)iz * L.step));
+  } else {
+    p = make_float3(pts[3 * (size_t)q], pts[3 * (size_t)q + 1], pts[3 * (size_t)q + 2]);
+  }
+  out[q] = density_point(p, d, c, g);
 }
 
 // ------------------------------------------------- persistent schedule --
@@ -931,7 +962,7 @@ hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx) {
 
 // Load-time density: hash the committed positions (no move), sort, then sum
 // poly6 over the 27 cells.  Leaves the state re-sorted (posB/velB hold it).
-hipError_t launch_densities(Ctx& x, const SimConst& c, float* out) {
+hipError_t launch_sort_committed(Ctx& x, const SimConst& c) {
   hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
   if (e != hipSuccess) return e;
   const unsigned nb = blocks_for(x.n);
@@ -942,7 +973,27 @@ hipError_t launch_densities(Ctx& x, const SimConst& c, float* out) {
     k_predict<false, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0]);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_scan(x)) != hipSuccess) return e;
-  if ((e = launch_scatter_rank(x)) != hipSuccess) return e;
+  return launch_scatter_rank(x);
+}
+
+hipError_t launch_density_field(Ctx& x, const SimConst& c, int scene, const float* pts,
+                                const DensityLattice* lattice, uint32_t m, float* out) {
+  if (m == 0) return hipSuccess;
+  const Dev d = dev_of(x);
+  const SceneGrid& g = x.grids[(size_t)scene];
+  const unsigned nb = (unsigned)((m + kBlock - 1) / kBlock);
+  if (lattice)
+    k_density_field<true><<<nb, kBlock, 0, x.stream>>>(d, c, g, nullptr, *lattice, m, out);
+  else
+    k_density_field<false><<<nb, kBlock, 0, x.stream>>>(d, c, g, pts, DensityLattice{}, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_densities(Ctx& x, const SimConst& c, float* out) {
+  hipError_t e = launch_sort_committed(x, c);
+  if (e != hipSuccess) return e;
+  const unsigned nb = blocks_for(x.n);
+  Dev d = dev_of(x);
   if (multi(x))
     k_densities<true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
   else

@@ -9,7 +9,12 @@
 // reference's CPU path tracer; this driver writes the final particles back in
 // the same XML format instead (-o), for the CPU consumers.
 //
-//   pbf_driver -p particles.xml [-d seconds] [-t dt] [-o out.xml] [-q]
+// Checkpoint/resume: -s writes a binary snapshot of the final state, and
+// -r resumes from one (the particles of -p then only give the rest
+// density, which must be the snapshot's).
+//
+//   pbf_driver -p particles.xml [-d seconds] [-t dt] [-o out.xml] [-s out.snap]
+//              [-r in.snap] [-q]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -109,6 +114,8 @@ int save_xml(const char* path, double rho0, pbf::ParticlesT<Vec3>& ps) {
 int main(int argc, char** argv) {
   const char* in = nullptr;
   const char* out = nullptr;
+  const char* snap_out = nullptr;
+  const char* snap_in = nullptr;
   double seconds = 1.0, dt = 0.0;
   bool quiet = false;
   for (int i = 1; i < argc; i++) {
@@ -116,9 +123,13 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(argv[i], "-d") && i + 1 < argc) seconds = std::atof(argv[++i]);
     else if (!std::strcmp(argv[i], "-t") && i + 1 < argc) dt = std::atof(argv[++i]);
     else if (!std::strcmp(argv[i], "-o") && i + 1 < argc) out = argv[++i];
+    else if (!std::strcmp(argv[i], "-s") && i + 1 < argc) snap_out = argv[++i];
+    else if (!std::strcmp(argv[i], "-r") && i + 1 < argc) snap_in = argv[++i];
     else if (!std::strcmp(argv[i], "-q")) quiet = true;
     else {
-      std::fprintf(stderr, "usage: %s -p particles.xml [-d seconds] [-t dt] [-o out.xml] [-q]\n",
+      std::fprintf(stderr,
+                   "usage: %s -p particles.xml [-d seconds] [-t dt] [-o out.xml] [-s out.snap] "
+                   "[-r in.snap] [-q]\n",
                    argv[0]);
       return 2;
     }
@@ -133,9 +144,15 @@ int main(int argc, char** argv) {
   if (rc) return rc;
   try {
     pbf::ParticlesT<Vec3> particles(rho0, nullptr, quiet);
-    for (size_t i = 0; i < pos.size(); i++) particles.addParticle(pos[i], vel[i]);
-    particles.estimateDensities();
-    std::printf("[Fluid Simulation] %zu particles, rest density %g\n", pos.size(), rho0);
+    if (snap_in) {
+      particles.loadSnapshot(snap_in);
+      std::printf("[Fluid Simulation] resumed %zu particles at t=%.6f from %s\n",
+                  particles.size(), particles.simulate_time, snap_in);
+    } else {
+      for (size_t i = 0; i < pos.size(); i++) particles.addParticle(pos[i], vel[i]);
+      particles.estimateDensities();
+      std::printf("[Fluid Simulation] %zu particles, rest density %g\n", pos.size(), rho0);
+    }
     const double start = particles.simulate_time;
     int steps = 0;
     while (particles.simulate_time < start + seconds) {
@@ -154,8 +171,10 @@ int main(int argc, char** argv) {
       const Vec3 p = particles.getPosition(i);
       cx += p.x; cy += p.y; cz += p.z;
     }
+    const double np = (double)particles.size();
     std::printf("done: %d steps, simulate_time %.6f, centroid %.9g %.9g %.9g\n", steps,
-                particles.simulate_time, cx / pos.size(), cy / pos.size(), cz / pos.size());
+                particles.simulate_time, cx / np, cy / np, cz / np);
+    if (snap_out) particles.saveSnapshot(snap_out);
     if (out && save_xml(out, rho0, particles) != 0) {
       std::fprintf(stderr, "[ERROR] cannot write %s\n", out);
       return 1;

@@ -9,6 +9,7 @@ read-back is in load order.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -110,6 +111,35 @@ class Particles:
         rho = np.empty(self.n, np.float32)
         check(lib().pbf_estimate_densities(self._ctx, _fptr(rho)), self._ctx)
         return rho
+
+    def save_snapshot(self, path: str) -> None:
+        """Binary checkpoint of the committed state (x, v, rho, simulate_time)."""
+        check(lib().pbf_save_snapshot(self._ctx, os.fsencode(path)), self._ctx)
+
+    def load_snapshot(self, path: str) -> None:
+        """Resume from a checkpoint written with the same parameters/scenes."""
+        check(lib().pbf_load_snapshot(self._ctx, os.fsencode(path)), self._ctx)
+        self.n = lib().pbf_count(self._ctx)
+
+    def density_at(self, pts, scene: int = 0) -> np.ndarray:
+        """Density of the particles at query points pts[m, 3]
+        (estimateDensityAt, src/particles.cpp:446-453)."""
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.empty(len(pts), np.float32)
+        check(lib().pbf_density_at(self._ctx, scene, _fptr(pts), len(pts), _fptr(out)), self._ctx)
+        return out
+
+    def density_lattice(self, lo, hi, step: float, counts, scene: int = 0) -> np.ndarray:
+        """Density on the lattice (min(hi, lo + i*step)), shape
+        (counts[2], counts[1], counts[0])."""
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        cnt = np.ascontiguousarray(counts, np.int32)
+        out = np.empty(int(np.prod(cnt.astype(np.int64))), np.float32)
+        check(lib().pbf_density_lattice(self._ctx, scene, _fptr(lo), _fptr(hi), float(step),
+                                        cnt.ctypes.data_as(C.POINTER(C.c_int32)), _fptr(out)),
+              self._ctx)
+        return out.reshape(int(cnt[2]), int(cnt[1]), int(cnt[0]))
 
     def time_step_profiled(self, nsteps: int = 1, dt: float | None = None) -> dict:
         """Steps like time_step and returns {pass: (total ms, launches)} from

@@ -132,6 +132,41 @@ PBF_API int pbf_read(pbf_ctx *ctx, float *xyz, float *v, float *rho);
  * evaluated on the grid.  Writes n floats in load order. */
 PBF_API int pbf_estimate_densities(pbf_ctx *ctx, float *rho);
 
+/* Checkpoint / resume.  The reference keeps state only in its XML input
+ * (src/application.cpp:302-344, ~100 bytes of text per particle); a binary
+ * snapshot restores a run exactly: loading a snapshot into a context made
+ * with the same parameters (and, batched, the same scenes) and stepping on
+ * gives the bits of the uninterrupted run.  File layout (little-endian):
+ * a 64-byte header {char magic[8] = "PBFSNAP1"; uint32 header_bytes = 64;
+ * uint32 params_bytes = sizeof(pbf_params); int64 n; int32 nscenes;
+ * int32 0; double simulate_time; uint64 FNV-1a-64 of the payload;
+ * 16 zero bytes}, then the payload: pbf_params, int64 count[nscenes],
+ * float box_lo[3], box_hi[3] per scene, float x[n*3], v[n*3], rho[n]
+ * (load order).  Loading checks magic, sizes, checksum, parameters and
+ * scenes (PBF_ERR_INVALID on any mismatch) and restores simulate_time. */
+PBF_API int pbf_save_snapshot(pbf_ctx *ctx, const char *path);
+PBF_API int pbf_load_snapshot(pbf_ctx *ctx, const char *path);
+
+/* Density field of the committed particles, the reference's
+ * Particles::estimateDensityAt (src/particles.cpp:446-453) as called by the
+ * marching-cubes consumers generateGridCell (:309-323, 8 corners per cube)
+ * and getVertexNormal (:404-418, 6 points per vertex).  out[q] = sum over
+ * the particles of scene `scene` (0 unless batched) of poly6(x_j - p_q),
+ * evaluated on the cell grid: particles of the 27 cells around p_q in
+ * (cell, id) order.  pts: m*3 floats.  Like pbf_estimate_densities it
+ * re-sorts the committed state (the positions do not change). */
+PBF_API int pbf_density_at(pbf_ctx *ctx, int32_t scene, const float *pts, int64_t m, float *out);
+
+/* The same field on the lattice of marching-cubes corners: point (ix, iy,
+ * iz) is (min(hi[0], lo[0] + ix*step), min(hi[1], lo[1] + iy*step),
+ * min(hi[2], lo[2] + iz*step)) in float, for i < counts[axis]; out is
+ * [counts[2]][counts[1]][counts[0]] (x fastest).  The reference's
+ * getSurfacePrims (:350-390) samples corners xmin + ix*fStepSize and
+ * min(xmax, x1 + fStepSize) for ix <= (int)((xmax-xmin)/fStepSize), i.e.
+ * counts = (int)((max-min)/step) + 2 per axis. */
+PBF_API int pbf_density_lattice(pbf_ctx *ctx, int32_t scene, const float lo[3], const float hi[3],
+                                float step, const int32_t counts[3], float *out);
+
 /* Particle::simulate_time of the reference (src/particles.h:109). */
 PBF_API double pbf_simulate_time(const pbf_ctx *ctx);
 PBF_API int64_t pbf_count(const pbf_ctx *ctx);

@@ -86,23 +86,61 @@ class ParticlesT {
     check(pbf_estimate_densities(ctx_, rho_.data()), ctx_);
   }
 
-  // SPH density at an arbitrary point over the read-back positions
-  // (src/particles.cpp:446-453); a CPU consumer, like marching cubes.
+  // Checkpoint / resume through binary snapshots (pbf_save_snapshot).  A
+  // context built with the same parameters resumes bit for bit.
+  void saveSnapshot(const char* path) {
+    upload();
+    check(pbf_save_snapshot(ctx_, path), ctx_);
+  }
+  void loadSnapshot(const char* path) {
+    check(pbf_load_snapshot(ctx_, path), ctx_);
+    const size_t n = (size_t)pbf_count(ctx_);
+    x_.assign(3 * n, 0.0f);
+    v_.assign(3 * n, 0.0f);
+    rho_.assign(n, 0.0f);
+    simulate_time = pbf_simulate_time(ctx_);
+    dirty_ = false;
+    stale_ = true;
+    surfaceUpToTimestep = false;
+  }
+
+  // SPH density at an arbitrary point (src/particles.cpp:446-453), on the
+  // GPU over the cell grid of the committed state.  One call per point works
+  // (the grid is built once per state), but the marching-cubes consumers
+  // should ask for all their points at once (estimateDensitiesAt,
+  // densityLattice).
   double estimateDensityAt(Vec3 pos) {
-    sync();
-    const double h = p_.h, h2 = h * h;
-    double h9 = h;
-    for (int k = 1; k < 9; k++) h9 *= h;
-    double density = 0.0;
-    for (size_t i = 0; i < size(); i++) {
-      const double dx = x_[3 * i] - pos.x, dy = x_[3 * i + 1] - pos.y, dz = x_[3 * i + 2] - pos.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < h2) {
-        const double t = h2 - r2;
-        density += (double)p_.poly6_coef * (t * t * t) / h9;
-      }
+    const float q[3] = {(float)pos.x, (float)pos.y, (float)pos.z};
+    float out = 0.0f;
+    upload();
+    check(pbf_density_at(ctx_, 0, q, 1, &out), ctx_);
+    return out;
+  }
+
+  // The density at many points: generateGridCell's 8 corners
+  // (src/particles.cpp:309-323) and getVertexNormal's 6 samples (:404-418).
+  std::vector<double> estimateDensitiesAt(const std::vector<Vec3>& pts) {
+    std::vector<float> q(3 * pts.size()), out(pts.size());
+    for (size_t i = 0; i < pts.size(); i++) {
+      q[3 * i] = (float)pts[i].x;
+      q[3 * i + 1] = (float)pts[i].y;
+      q[3 * i + 2] = (float)pts[i].z;
     }
-    return density;
+    upload();
+    check(pbf_density_at(ctx_, 0, q.data(), (int64_t)pts.size(), out.data()), ctx_);
+    return std::vector<double>(out.begin(), out.end());
+  }
+
+  // The density on getSurfacePrims' corner lattice (src/particles.cpp:
+  // 350-390): counts[a] = (int)((max-min)/step) + 2 points per axis at
+  // min(max, min + i*step); x fastest.
+  std::vector<float> densityLattice(Vec3 lo, Vec3 hi, double step, int32_t counts[3]) {
+    const float l[3] = {(float)lo.x, (float)lo.y, (float)lo.z};
+    const float u[3] = {(float)hi.x, (float)hi.y, (float)hi.z};
+    std::vector<float> out((size_t)counts[0] * (size_t)counts[1] * (size_t)counts[2]);
+    upload();
+    check(pbf_density_lattice(ctx_, 0, l, u, (float)step, counts, out.data()), ctx_);
+    return out;
   }
 
   std::string paramsString() const {

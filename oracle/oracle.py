@@ -281,5 +281,16 @@ def estimate_densities(p, x, mode=NBR_GRID, dtype=np.float32):
     return rho
 
 
+def density_at(p, x, pts, mode=NBR_GRID, dtype=np.float32):
+    """Density of particles x at query points pts (estimateDensityAt)."""
+    x = _arr(x, dtype)
+    pts = _arr(pts, dtype)
+    out = np.zeros(len(pts), dtype)
+    op = params(p, dtype == np.float32)
+    getattr(lib(), "orc_density_at_" + _t(dtype))(C.byref(op), int(mode), C.c_size_t(len(x)), _ptr(x),
+                                                  C.c_size_t(len(pts)), _ptr(pts), _ptr(out))
+    return out
+
+
 def csr_to_lists(off, lst):
     return [lst[off[i]:off[i + 1]] for i in range(len(off) - 1)]

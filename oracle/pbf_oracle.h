@@ -133,6 +133,14 @@ int orc_step_d(const orc_params *p, int nbr_mode, int xsph_mode, size_t n,
 void orc_estimate_densities_f(const orc_params *p, int mode, size_t n, const float *x, float *rho);
 void orc_estimate_densities_d(const orc_params *p, int mode, size_t n, const double *x, double *rho);
 
+/* Density field at m query points pts[m*3] (Particles::estimateDensityAt,
+ * src/particles.cpp:446-453; self not special: a query at a particle's
+ * position includes its W(0)). */
+void orc_density_at_f(const orc_params *p, int mode, size_t n, const float *x, size_t m,
+                      const float *pts, float *out);
+void orc_density_at_d(const orc_params *p, int mode, size_t n, const double *x, size_t m,
+                      const double *pts, double *out);
+
 /* Threads used by the OpenMP loops (0 = runtime default). */
 void orc_set_threads(int n);
 int  orc_get_threads(void);

@@ -690,6 +690,36 @@ void FN(orc_estimate_densities_)(const orc_params *p, int mode, size_t n, const 
   FN(grid_free_)(&g);
 }
 
+/* estimateDensityAt, src/particles.cpp:446-453: density of the particles at
+ * m query points, sum_j poly6(x_j - p).  BRUTE sums over every particle in
+ * ascending j (the reference's order); GRID sums over the 27 cells around
+ * p in (cell, index) order.  Queries outside the box take the clamped
+ * boundary cell, whose neighbourhood still holds every particle within h. */
+void FN(orc_density_at_)(const orc_params *p, int mode, size_t n, const REAL *x, size_t m,
+                         const REAL *pts, REAL *out) {
+  DER d;
+  FN(der_init_)(p, (REAL)p->dt, &d);
+  if (mode == ORC_NBR_BRUTE) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long q = 0; q < (long)m; q++) {
+      REAL s = 0;
+      for (size_t j = 0; j < n; j++) s = s + FN(poly6_)(&d, FN(dist2_)(x + 3 * j, pts + 3 * q));
+      out[q] = s;
+    }
+    return;
+  }
+  CAT(orc_grid, SFX) g;
+  if (FN(grid_build_)(&d, n, x, &g) != 0) return;
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < (long)m; q++) {
+    REAL s = 0;
+    GRID_FOREACH(&d, &g, pts + 3 * q, j,
+                 { s = s + FN(poly6_)(&d, FN(dist2_)(x + 3 * (size_t)j, pts + 3 * q)); });
+    out[q] = s;
+  }
+  FN(grid_free_)(&g);
+}
+
 #undef DER
 #undef FN
 #undef CAT

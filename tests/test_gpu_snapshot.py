@@ -1,0 +1,94 @@
+"""Checkpoint / resume through the C ABI's binary snapshots: a resumed run
+gives the bits of the uninterrupted one, and damaged or mismatched files are
+refused."""
+import os
+
+import numpy as np
+import pytest
+
+from fluid_amd import scenes
+from fluid_amd.pbf import Particles, PbfError
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_resume_is_bit_exact(tmp_path):
+    sc = scenes.dam_block()
+    path = str(tmp_path / "run.pbfsnap")
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.time_step(nsteps=6)
+        c.save_snapshot(path)
+        mid = c.read()
+        t_mid = c.simulate_time
+        c.time_step(nsteps=6)
+        whole = c.read()
+        t_end = c.simulate_time
+    n = len(sc.x)
+    assert os.path.getsize(path) == 64 + 92 + 8 + 24 + 4 * 7 * n  # header, params, count, box, x v rho
+    with Particles(sc.params) as r:
+        r.load_snapshot(path)
+        assert r.n == n and r.simulate_time == t_mid
+        for a, b in zip(r.read(), mid):
+            assert bits_equal(a, b)  # x, v and the last densities
+        r.time_step(nsteps=6)
+        assert r.simulate_time == t_end
+        for a, b in zip(r.read(), whole):
+            assert bits_equal(a, b)
+
+
+def test_batched_resume_is_bit_exact(tmp_path):
+    items, p = scenes.batched(nscenes=5, nmin=500, nmax=2000)
+    spec = [(len(x), lo, hi) for x, _, lo, hi in items]
+    x0 = np.concatenate([it[0] for it in items])
+    v0 = np.concatenate([it[1] for it in items])
+    path = str(tmp_path / "batch.pbfsnap")
+    with Particles(p, scenes=spec) as b:
+        b.load(x0, v0)
+        b.time_step(nsteps=4)
+        b.save_snapshot(path)
+        b.time_step(nsteps=4)
+        whole = b.read()
+    with Particles(p, scenes=spec) as r:
+        r.load_snapshot(path)
+        r.time_step(nsteps=4)
+        for a, b_ in zip(r.read(), whole):
+            assert bits_equal(a, b_)
+    # a context with other scenes refuses it
+    with Particles(p, scenes=spec[:-1] + [(spec[-1][0], spec[-1][1], np.asarray(spec[-1][2]) + 1)]) as o:
+        with pytest.raises(PbfError, match="boxes"):
+            o.load_snapshot(path)
+
+
+def test_damaged_or_foreign_snapshots_are_refused(tmp_path):
+    sc = scenes.dam_block()
+    path = str(tmp_path / "a.pbfsnap")
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.time_step(nsteps=1)
+        c.save_snapshot(path)
+    raw = open(path, "rb").read()
+    cases = {
+        "flip": raw[:200] + bytes([raw[200] ^ 1]) + raw[201:],   # payload bit: checksum
+        "short": raw[:-4],                                        # truncated
+        "long": raw + b"\0",                                      # trailing bytes
+        "magic": b"XBFSNAP1" + raw[8:],                           # not a snapshot
+    }
+    for name, data in cases.items():
+        bad = str(tmp_path / f"{name}.pbfsnap")
+        open(bad, "wb").write(data)
+        with Particles(sc.params) as r:
+            with pytest.raises(PbfError):
+                r.load_snapshot(bad)
+    with Particles(sc.params.replace(k_tensile=sc.params.k_tensile * 2)) as r:
+        with pytest.raises(PbfError, match="parameters"):
+            r.load_snapshot(path)
+    with Particles(sc.params) as r:
+        with pytest.raises(PbfError):
+            r.load_snapshot(str(tmp_path / "missing.pbfsnap"))

@@ -62,3 +62,20 @@ def test_driver_one_second_matches_oracle(tmp_path, orc):
     _, gx, gv = xmlio.load_particles(str(out))
     assert np.array_equal(gx.astype(np.float32), xr)
     assert np.array_equal(gv.astype(np.float32), vr)
+
+
+@pytest.mark.gpu
+def test_driver_checkpoint_resume_matches_one_run(tmp_path):
+    """-d 0.5 -s snap, then -r snap -d 0.49: the 32 + 31 steps give the same
+    file as one -d 1 run of 63 steps."""
+    src = os.path.join(GOLDEN, "p.xml")
+    one, two, snap = tmp_path / "one.xml", tmp_path / "two.xml", tmp_path / "mid.pbfsnap"
+    r = run("-p", src, "-d", "1.0", "-q", "-o", str(one))
+    assert r.returncode == 0, r.stderr
+    r = run("-p", src, "-d", "0.5", "-q", "-s", str(snap))
+    assert r.returncode == 0 and "done: 32 steps" in r.stdout, r.stderr
+    r = run("-p", src, "-r", str(snap), "-d", "0.49", "-q", "-o", str(two))
+    assert r.returncode == 0 and "done: 31 steps, simulate_time 1.008000" in r.stdout, r.stdout + r.stderr
+    assert one.read_bytes() == two.read_bytes()
+    r = run("-p", src, "-r", str(tmp_path / "missing.pbfsnap"))
+    assert r.returncode == 1 and "cannot open" in r.stderr

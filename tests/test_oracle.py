@@ -367,3 +367,60 @@ def test_load_time_density_grid_equals_brute(orc):
     p = sc.params
     r = orc.estimate_densities(p, np.array([[0.5, 0.5, 0.5]]), orc.NBR_BRUTE, np.float64)
     assert r[0] == pytest.approx(P.POLY6_LITERAL / 0.1 ** 3, rel=1e-12)
+
+
+# ------------------------------------------------------------ density field --
+def density_queries(rng, lo, hi, x, m):
+    """Query points: uniform in and around the box (some outside it), plus
+    some particle positions."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    pad = 0.15 * (hi - lo)
+    q = rng.uniform(lo - pad, hi + pad, size=(m, 3))
+    return np.concatenate([q, x[: m // 4]]).astype(np.float32)
+
+
+def test_density_field_equals_numpy_brute_force(orc):
+    """estimateDensityAt (src/particles.cpp:446-453) on the grid equals an
+    independent float64 sum over every particle (catches a missed cell of
+    the 27, a wrong clamp for points outside the box, or r = p - x_j sign
+    errors that would not matter for W but would in a different kernel)."""
+    rng = np.random.default_rng(11)
+    lo, hi = (0.0, 0.0, 0.0), (1.3, 0.9, 1.1)
+    p = P.scene_params(0.1, 0.05, lo, hi)
+    x = rng.uniform(lo, hi, size=(4000, 3)).astype(np.float32)
+    q = density_queries(rng, lo, hi, x, 2000)
+    got = orc.density_at(p, x, q, orc.NBR_GRID, np.float64)
+    h = 0.1
+    c1 = P.POLY6_LITERAL / h ** 9
+    want = np.zeros(len(q))
+    for k in range(0, len(q), 250):
+        r2 = ((x[None, :, :].astype(np.float64) - q[k:k + 250, None, :]) ** 2).sum(-1)
+        want[k:k + 250] = np.where(r2 < h * h, c1 * (h * h - r2) ** 3, 0.0).sum(1)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-9)
+    assert (want[:2000] == 0).any() and (want > 0).sum() > 1000  # both cases occur
+    f32 = orc.density_at(p, x, q, orc.NBR_GRID, np.float32)
+    assert np.allclose(f32, want, rtol=2e-5, atol=1e-3)
+    brute = orc.density_at(p, x, q, orc.NBR_BRUTE, np.float32)
+    assert np.allclose(brute, want, rtol=2e-5, atol=1e-3)
+
+
+def test_density_field_at_particles_is_the_load_time_density(orc):
+    """At a particle's own position the field is its load-time density,
+    self included, bit for bit: both sum the same 27 cells in the same
+    order with r = x_j - x_i."""
+    sc = scenes.dam_block()
+    rho = orc.estimate_densities(sc.params, sc.x, orc.NBR_GRID)
+    at = orc.density_at(sc.params, sc.x, sc.x, orc.NBR_GRID)
+    assert np.array_equal(rho.view(np.uint32), at.view(np.uint32))
+
+
+def test_density_field_single_particle_closed_form(orc):
+    """One particle: the field is c1 (h^2 - r^2)^3 / h^9 inside h and 0 at
+    and beyond h."""
+    p = P.scene_params(0.1, 0.05, (0, 0, 0), (1, 1, 1))
+    x = np.array([[0.5, 0.5, 0.5]])
+    r = np.array([0.0, 0.03, 0.07, 0.0999, 0.1, 0.2])
+    q = np.stack([0.5 + r, np.full_like(r, 0.5), np.full_like(r, 0.5)], 1)
+    got = orc.density_at(p, x, q, orc.NBR_GRID, np.float64)
+    want = np.where(r < 0.1, P.POLY6_LITERAL * (0.01 - r * r) ** 3 / 0.1 ** 9, 0.0)
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-12)
