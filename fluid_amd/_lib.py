@@ -57,6 +57,8 @@ SIGNATURES = {
     "pbf_estimate_densities": (C.c_int, [_P, _F]),
     "pbf_density_at": (C.c_int, [_P, C.c_int32, _F, C.c_int64, _F]),
     "pbf_save_snapshot": (C.c_int, [_P, C.c_char_p]),
+    "pbf_set_mesh": (C.c_int, [_P, _F, _F, C.c_int64, C.POINTER(C.c_int32), C.c_int64,
+                               C.c_float]),
     "pbf_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "pbf_density_lattice": (C.c_int, [_P, C.c_int32, _F, _F, C.c_float,
                                       C.POINTER(C.c_int32), _F]),

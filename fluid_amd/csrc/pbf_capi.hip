@@ -464,6 +464,8 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   drop_graph(x);
   free_device(x);
+  dfree(x->d.meshTri);
+  dfree(x->d.meshNode);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return PBF_OK;
@@ -739,6 +741,149 @@ PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
     HIPCHK(x, pbf::launch_unpermute(*x, d.posA, d.velA, 3, 1, d.stage));
     if ((rc = copy_out(x, d.stage, rho, n * sizeof(float))) != PBF_OK) return rc;
   }
+  return PBF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// ---- mesh collider: BVH build on the host --------------------------------
+// Median split on the longest axis of the centroid bounds (ties by index, so
+// the tree is deterministic), leaves of at most 4 triangles, depth at most
+// 40 (the device traversal stack holds 64 entries).  Node boxes are padded
+// outwards by 1e-5 of the scene scale so the float slab test is
+// conservative.
+struct MeshBuild {
+  const float* pos;
+  const int32_t* tri;
+  std::vector<uint32_t> order;        // triangle indices, leaf order
+  std::vector<float> cen, lo, hi;     // per triangle: centroid, box
+  std::vector<float4> nodes;          // 2 per node
+  float pad = 0.0f;
+  int depth = 0;
+
+  uint32_t build(uint32_t b, uint32_t e, int level) {
+    depth = std::max(depth, level);
+    const uint32_t ni = (uint32_t)(nodes.size() / 2);
+    nodes.resize(nodes.size() + 2);
+    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = b; i < e; i++) {
+      const uint32_t k = order[i];
+      for (int a = 0; a < 3; a++) {
+        bl[a] = std::min(bl[a], lo[3 * k + a]);
+        bh[a] = std::max(bh[a], hi[3 * k + a]);
+        cl[a] = std::min(cl[a], cen[3 * k + a]);
+        ch[a] = std::max(ch[a], cen[3 * k + a]);
+      }
+    }
+    uint32_t ua, ub;
+    if (e - b <= 4 || level >= 40) {
+      ua = b;
+      ub = 0x80000000u | (e - b);
+    } else {
+      int ax = 0;
+      for (int a = 1; a < 3; a++)
+        if (ch[a] - cl[a] > ch[ax] - cl[ax]) ax = a;
+      const uint32_t mid = b + (e - b) / 2;
+      std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
+                       [&](uint32_t x, uint32_t y) {
+                         const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
+                         return cx < cy || (cx == cy && x < y);
+                       });
+      ua = build(b, mid, level + 1);
+      ub = build(mid, e, level + 1);
+    }
+    float ufa, ufb;
+    memcpy(&ufa, &ua, 4);
+    memcpy(&ufb, &ub, 4);
+    nodes[2 * ni] = make_float4(bl[0] - pad, bl[1] - pad, bl[2] - pad, ufa);
+    nodes[2 * ni + 1] = make_float4(bh[0] + pad, bh[1] + pad, bh[2] + pad, ufb);
+    return ni;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t nv,
+                         const int32_t* tri, int64_t nt, float skin) {
+  Ctx* x = C(c);
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (!is_finite_f(skin) || skin < 0.0f) return fail(x, PBF_ERR_INVALID, "skin must be >= 0");
+  if (nt < 0 || nt >= ((int64_t)1 << 27)) return fail(x, PBF_ERR_INVALID, "0 <= nt < 2^27 required");
+  if (nt > 0 && x->nscenes != 1)
+    return fail(x, PBF_ERR_INVALID, "a mesh collider needs a single-scene context");
+  if (nt > 0 && (!pos || !nrm || !tri || nv < 3 || nv >= ((int64_t)1 << 31)))
+    return fail(x, PBF_ERR_INVALID, "pos, nrm, tri required and 3 <= nv < 2^31");
+  for (int64_t i = 0; nt > 0 && i < 3 * nv; i++)
+    if (!is_finite_f(pos[i]) || !is_finite_f(nrm[i]))
+      return fail(x, PBF_ERR_INVALID, "non-finite mesh vertex or normal");
+  for (int64_t i = 0; i < 3 * nt; i++)
+    if (tri[i] < 0 || tri[i] >= nv)
+      return fail(x, PBF_ERR_INVALID, "triangle " + std::to_string(i / 3) + " has a bad vertex index");
+  HIPCHK(x, hipSetDevice(x->device));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  drop_graph(x);
+  dfree(x->d.meshTri);
+  dfree(x->d.meshNode);
+  x->mesh_ntri = 0;
+  if (nt == 0) return PBF_OK;
+  MeshBuild mb;
+  mb.pos = pos;
+  mb.tri = tri;
+  const size_t n = (size_t)nt;
+  mb.order.resize(n);
+  mb.cen.resize(3 * n);
+  mb.lo.resize(3 * n);
+  mb.hi.resize(3 * n);
+  float scale = 0.0f;
+  for (size_t k = 0; k < n; k++) {
+    mb.order[k] = (uint32_t)k;
+    for (int a = 0; a < 3; a++) {
+      const float q0 = pos[3 * (size_t)tri[3 * k] + a], q1 = pos[3 * (size_t)tri[3 * k + 1] + a],
+                  q2 = pos[3 * (size_t)tri[3 * k + 2] + a];
+      mb.lo[3 * k + a] = std::min(q0, std::min(q1, q2));
+      mb.hi[3 * k + a] = std::max(q0, std::max(q1, q2));
+      mb.cen[3 * k + a] = (q0 + q1 + q2) / 3.0f;
+      scale = std::max(scale, std::max(fabsf(mb.lo[3 * k + a]), fabsf(mb.hi[3 * k + a])));
+    }
+  }
+  for (int a = 0; a < 3; a++)
+    scale = std::max(scale, std::max(fabsf(x->grids[0].lo[a]), fabsf(x->grids[0].hi[a])));
+  mb.pad = 1e-5f * std::max(scale, 1e-6f);
+  mb.build(0, (uint32_t)n, 0);
+  // Triangle records in leaf order: p1 (original index bits), e1, e2, n1..n3.
+  std::vector<float4> tv(6 * n);
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t k = mb.order[i];
+    const float* p1 = pos + 3 * (size_t)tri[3 * k];
+    const float* p2 = pos + 3 * (size_t)tri[3 * k + 1];
+    const float* p3 = pos + 3 * (size_t)tri[3 * k + 2];
+    float kf;
+    memcpy(&kf, &k, 4);
+    tv[6 * i] = make_float4(p1[0], p1[1], p1[2], kf);
+    tv[6 * i + 1] = make_float4(p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2], 0.0f);
+    tv[6 * i + 2] = make_float4(p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2], 0.0f);
+    for (int j = 0; j < 3; j++) {
+      const float* q = nrm + 3 * (size_t)tri[3 * k + j];
+      tv[6 * i + 3 + j] = make_float4(q[0], q[1], q[2], 0.0f);
+    }
+  }
+  hipError_t e = dalloc(x->d.meshTri, tv.size());
+  if (e == hipSuccess) e = dalloc(x->d.meshNode, mb.nodes.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(x->d.meshTri, tv.data(), tv.size() * sizeof(float4), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(x->d.meshNode, mb.nodes.data(), mb.nodes.size() * sizeof(float4),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dfree(x->d.meshTri);
+    dfree(x->d.meshNode);
+    return fail(x, PBF_ERR_NOMEM, std::string("mesh upload: ") + hipGetErrorString(e));
+  }
+  x->mesh_ntri = (uint32_t)nt;
+  x->mesh_skin = skin;
   return PBF_OK;
 }
 

@@ -77,16 +77,110 @@ __device__ __forceinline__ float comp(const float3& v, int a) {
   return a == 0 ? v.x : (a == 1 ? v.y : v.z);
 }
 
+// Triangle-mesh collider (the reference's scene BVH, src/particles.cpp:51-132
+// with src/bvh.cpp:142-192 and src/static_scene/triangle.cpp:21-80).
+// Triangle k: tri[6k] = (p1, original index bits), tri[6k+1] = e1 = p2 - p1,
+// tri[6k+2] = e2 = p3 - p1, tri[6k+3..5] = vertex normals n1, n2, n3.
+// BVH node: node[2i] = (lo, a), node[2i+1] = (hi, b) with a, b as integer
+// bits; b >= 2^31 marks a leaf of triangles [a, a + (b & 0x7fffffff)),
+// otherwise a and b are the children.  Node boxes are padded outwards, so
+// the box test never drops a triangle that the exact test would accept.
+struct MeshView {
+  const float4* tri = nullptr;
+  const float4* node = nullptr;
+  uint32_t ntri = 0;  // 0: the analytic box walls
+  float skin = 0.0f;  // clearance a hit leaves along the face normal
+};
+
+__device__ __forceinline__ float3 f3(const float4& a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float dot3(float3 u, float3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
+__device__ __forceinline__ float3 cross3(float3 u, float3 v) {
+  return make_float3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+
+// Nearest triangle hit of the segment o + t d, 0 <= t <= tmax: the smallest
+// t, ties to the lowest original triangle index, so the result does not
+// depend on the traversal order.  Each triangle test is Triangle::intersect
+// in float with the reference's operation order.  Returns the triangle's
+// original index or -1; *t, *n (w n1 + u n2 + v n3) and the face normal
+// *nf = e1 x e2 on a hit.
+__device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float tmax, float* t_out,
+                                   float3* n_out, float3* nf_out) {
+  float bt = tmax;
+  int best = -1;
+  uint32_t stack[64];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const uint32_t ni = stack[--sp];
+    const float4 lo = m.node[2 * ni], hi = m.node[2 * ni + 1];
+    // Slab test against [0, bt]; an axis the segment runs parallel to only
+    // needs the origin inside the slab.
+    float t0 = 0.0f, t1 = bt;
+    bool miss = false;
+    const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+    const float la[3] = {lo.x, lo.y, lo.z}, ha[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      if (da[a] == 0.0f) {
+        miss = miss || oa[a] < la[a] || oa[a] > ha[a];
+      } else {
+        const float inv = 1.0f / da[a];
+        float ta = (la[a] - oa[a]) * inv, tb = (ha[a] - oa[a]) * inv;
+        if (ta > tb) {
+          const float tt = ta;
+          ta = tb;
+          tb = tt;
+        }
+        t0 = fmaxf(t0, ta);
+        t1 = fminf(t1, tb);
+      }
+    }
+    // Rounding of the slab bounds is covered by the padded boxes; accept
+    // a box whose entry is within a relative 1e-6 of the current bound.
+    if (miss || t0 > t1 * 1.000001f + 1e-30f) continue;
+    const uint32_t a = __float_as_uint(lo.w), b = __float_as_uint(hi.w);
+    if (b & 0x80000000u) {
+      const uint32_t end = a + (b & 0x7fffffffu);
+      for (uint32_t k = a; k < end; k++) {
+        const float4 P = m.tri[6 * (size_t)k];
+        const float3 e1 = f3(m.tri[6 * (size_t)k + 1]), e2 = f3(m.tri[6 * (size_t)k + 2]);
+        const float3 sv = make_float3(o.x - P.x, o.y - P.y, o.z - P.z);
+        const float3 s1 = cross3(d, e2), s2 = cross3(sv, e1);
+        const float det = dot3(s1, e1);
+        if (det == 0.0f) continue;
+        const float t = dot3(s2, e2) / det;
+        const int idx = (int)__float_as_uint(P.w);
+        if (t < 0.0f || t > bt || (t == bt && best >= 0 && idx > best)) continue;
+        const float u = dot3(s1, sv) / det;
+        const float v = dot3(s2, d) / det;
+        const float w = (1.0f - u) - v;
+        if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || w < 0.0f || w > 1.0f) continue;
+        const float3 n1 = f3(m.tri[6 * (size_t)k + 3]), n2 = f3(m.tri[6 * (size_t)k + 4]),
+                     n3 = f3(m.tri[6 * (size_t)k + 5]);
+        bt = t;
+        best = idx;
+        *n_out = make_float3((w * n1.x + u * n2.x) + v * n3.x, (w * n1.y + u * n2.y) + v * n3.y,
+                             (w * n1.z + u * n2.z) + v * n3.z);
+        *nf_out = cross3(e1, e2);
+      }
+    } else if (sp + 2 <= 64) {
+      stack[sp++] = b;
+      stack[sp++] = a;
+    }
+  }
+  *t_out = bt;
+  return best;
+}
+
 // Box collision along dir for a displacement D of length l (= total).
 // Analytic stop planes first (strict 0 < t < l, as src/particles.cpp:59-74),
 // then the nearest mesh wall (0 <= t <= l, as the BVH triangle test); on a hit
 // move to t - EPS_D, and in prediction slide once along a mesh wall for half
 // the remaining length (:114-124).  Ends with the per-axis clamp (:129-131).
-__device__ __forceinline__ float3 collide(const SceneGrid& g, const SimConst& c, float3 p,
-                                          float3 D, float3 dir, float l, float total,
-                                          bool slide) {
-  float pa[3] = {p.x, p.y, p.z};
-  const float da[3] = {dir.x, dir.y, dir.z};
+// Analytic stop planes of the box (strict 0 < t < l, src/particles.cpp:59-74).
+__device__ __forceinline__ bool stop_planes(const SceneGrid& g, const float* pa, const float* da,
+                                            float* l) {
   bool intersect = false;
 #pragma unroll
   for (int a = 2; a >= 0; a--) {
@@ -95,11 +189,84 @@ __device__ __forceinline__ float3 collide(const SceneGrid& g, const SimConst& c,
     if (!((g.stop >> (2 * a + hw)) & 1u)) continue;
     float plane = hw ? g.hi[a] : g.lo[a];
     float pt = (plane - pa[a]) / da[a];
-    if (pt > 0.0f && pt < l) {
-      l = pt;
+    if (pt > 0.0f && pt < *l) {
+      *l = pt;
       intersect = true;
     }
   }
+  return intersect;
+}
+
+// After a hit on a face with normal nf, step off it by `skin` along the
+// unit normal, on the side the segment came from.  EPS_D (1e-11) is far
+// below float resolution: the reference's back-off along the ray alone can
+// leave a float position on or behind the face, from where the next move
+// passes through it.
+__device__ __forceinline__ void push_off(float* pa, float3 d, float3 nf, float skin) {
+  if (!(skin > 0.0f)) return;
+  const float dn = dot3(d, nf);
+  const float k = skin / sqrtf(dot3(nf, nf));
+  const float s = dn > 0.0f ? -k : k;
+  pa[0] = pa[0] + s * nf.x;
+  pa[1] = pa[1] + s * nf.y;
+  pa[2] = pa[2] + s * nf.z;
+}
+
+// Mesh collision: the stop planes, then the nearest triangle within the
+// (shortened) segment; on a hit move to t - EPS_D (and off the face by the
+// skin) and, in prediction, slide once along (D - (D.n) n).unit() for half
+// the remaining length, against the mesh only (src/particles.cpp:111-124).
+// Ends with the per-axis clamp.
+__device__ inline float3 collide_mesh(const SceneGrid& g, const SimConst& c, const MeshView& m,
+                                      float3 p, float3 D, float3 dir, float l, float total,
+                                      bool slide) {
+  float pa[3] = {p.x, p.y, p.z};
+  const float da[3] = {dir.x, dir.y, dir.z};
+  const bool intersect = stop_planes(g, pa, da, &l);
+  float th;
+  float3 n = make_float3(0.0f, 0.0f, 0.0f), nf = n;
+  const int hit = mesh_nearest(m, p, dir, l, &th, &n, &nf);
+  if (hit >= 0) l = th;
+  if (hit >= 0 || intersect) {
+    const float s = l - c.eps_d;
+    pa[0] = pa[0] + s * da[0];
+    pa[1] = pa[1] + s * da[1];
+    pa[2] = pa[2] + s * da[2];
+    if (hit >= 0) push_off(pa, dir, nf, m.skin);
+    if (slide && hit >= 0 && !intersect && dot3(dir, n) > -1.0f) {
+      const float dn = dot3(D, n);
+      const float T[3] = {D.x - dn * n.x, D.y - dn * n.y, D.z - dn * n.z};
+      const float tl2 = T[0] * T[0] + T[1] * T[1] + T[2] * T[2];
+      if (tl2 > 0.0f) {
+        const float rn = 1.0f / sqrtf(tl2);
+        const float3 d2 = make_float3(rn * T[0], rn * T[1], rn * T[2]);
+        float l2 = (total - l) * 0.5f, t2;
+        float3 n2, nf2;
+        const int hit2 = mesh_nearest(m, make_float3(pa[0], pa[1], pa[2]), d2, l2, &t2, &n2, &nf2);
+        if (hit2 >= 0) l2 = t2;
+        const float s2 = l2 - c.eps_d;
+        pa[0] = pa[0] + s2 * d2.x;
+        pa[1] = pa[1] + s2 * d2.y;
+        pa[2] = pa[2] + s2 * d2.z;
+        if (hit2 >= 0) push_off(pa, d2, nf2, m.skin);
+      }
+    }
+  } else {
+    pa[0] = pa[0] + D.x;
+    pa[1] = pa[1] + D.y;
+    pa[2] = pa[2] + D.z;
+  }
+  return make_float3(smax(g.lo_e[0], smin(g.hi_e[0], pa[0])),
+                     smax(g.lo_e[1], smin(g.hi_e[1], pa[1])),
+                     smax(g.lo_e[2], smin(g.hi_e[2], pa[2])));
+}
+
+__device__ __forceinline__ float3 collide(const SceneGrid& g, const SimConst& c, float3 p,
+                                          float3 D, float3 dir, float l, float total,
+                                          bool slide) {
+  float pa[3] = {p.x, p.y, p.z};
+  const float da[3] = {dir.x, dir.y, dir.z};
+  const bool intersect = stop_planes(g, pa, da, &l);
   int hit = -1;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
@@ -152,23 +319,25 @@ __device__ __forceinline__ float3 collide(const SceneGrid& g, const SimConst& c,
 }
 
 // clamp_response (src/particles.cpp:87-132): prediction with slide.
-__device__ __forceinline__ float3 clamp_response(const SceneGrid& g, const SimConst& c, float3 p,
-                                                 float3 v) {
+__device__ __forceinline__ float3 clamp_response(const SceneGrid& g, const SimConst& c,
+                                                 const MeshView& m, float3 p, float3 v) {
   float3 D = make_float3(v.x * c.dt, v.y * c.dt, v.z * c.dt);
   float total = sqrtf(D.x * D.x + D.y * D.y + D.z * D.z);
   if (total <= c.eps_d) return p;
   float rl = 1.0f / total;
   float3 dir = make_float3(rl * D.x, rl * D.y, rl * D.z);
+  if (m.ntri) return collide_mesh(g, c, m, p, D, dir, total, total, true);
   return collide(g, c, p, D, dir, total, total, true);
 }
 
 // clamp (src/particles.cpp:51-84): position correction, no slide.
-__device__ __forceinline__ float3 clamp_dp(const SceneGrid& g, const SimConst& c, float3 p,
-                                           float3 dp) {
+__device__ __forceinline__ float3 clamp_dp(const SceneGrid& g, const SimConst& c,
+                                           const MeshView& m, float3 p, float3 dp) {
   float l = sqrtf(dp.x * dp.x + dp.y * dp.y + dp.z * dp.z);
   if (l <= c.eps_d) return p;
   float rl = 1.0f / l;
   float3 dir = make_float3(rl * dp.x, rl * dp.y, rl * dp.z);
+  if (m.ntri) return collide_mesh(g, c, m, p, dp, dir, l, l, false);
   return collide(g, c, p, dp, dir, l, l, false);
 }
 

@@ -48,6 +48,8 @@ struct DevState {
   uint32_t* slotScene = nullptr;  // scene of each sorted slot (multi-scene)
   SceneRun* runs = nullptr;       // scenes, largest first (persistent schedule)
   float* stage = nullptr;         // read-back staging, n * 4 floats
+  float4* meshTri = nullptr;      // collider triangles, 6 float4 each (pbf_device.h MeshView)
+  float4* meshNode = nullptr;     // collider BVH, 2 float4 per node
 };
 
 struct Ctx {
@@ -70,6 +72,8 @@ struct Ctx {
   bool debug_dirty = false;
   bool has_nbrs = false;
   bool field_ready = false;          // committed state sorted by cell, cell table current
+  uint32_t mesh_ntri = 0;            // triangles of the mesh collider (0: box walls)
+  float mesh_skin = 0.0f;            // clearance a mesh hit leaves along the face normal
   const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop
   const float4* dbg_vel = nullptr;   // velocity buffer current at the last debug stop
   int schedule = PBF_SCHED_FLAT;     // PBF_SCHED_*

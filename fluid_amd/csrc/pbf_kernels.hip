@@ -48,6 +48,7 @@ struct Dev {
   uint32_t *cell, *arrival, *cellSorted, *tmpId, *cellCount, *cellStart, *nbr, *ncnt, *flags;
   const SceneGrid* grids;
   const uint32_t* slotScene;
+  MeshView mesh;  // triangle-mesh collider (ntri 0: the analytic box walls)
   uint32_t n;
   int K4;
 };
@@ -60,6 +61,10 @@ Dev dev_of(const Ctx& x) {
   d.cell = s.cell; d.arrival = s.arrival; d.cellSorted = s.cellSorted; d.tmpId = s.tmpId;
   d.cellCount = s.cellCount; d.cellStart = s.cellStart; d.nbr = s.nbr; d.ncnt = s.ncnt;
   d.flags = s.flags; d.grids = s.grids; d.slotScene = s.slotScene;
+  d.mesh.tri = s.meshTri;
+  d.mesh.node = s.meshNode;
+  d.mesh.ntri = x.mesh_ntri;
+  d.mesh.skin = x.mesh_skin;
   d.n = (uint32_t)x.n;
   d.K4 = k4_of(x);
   return d;
@@ -239,7 +244,7 @@ __device__ __forceinline__ void predict_one(uint32_t i, const Dev& d, const SimC
   if (MOVE) {
     float4 v = d.velA[i];
     v.y = v.y - c.gdt;
-    p = clamp_response(g, c, p, make_float3(v.x, v.y, v.z));
+    p = clamp_response(g, c, d.mesh, p, make_float3(v.x, v.y, v.z));
     d.velA[i] = v;
   }
   d.predP[i] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -449,7 +454,8 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   dx = dx * c.inv_rho0;
   dy = dy * c.inv_rho0;
   dz = dz * c.inv_rho0;
-  const float3 p = clamp_dp(g, c, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
+  const float3 p =
+      clamp_dp(g, c, d.mesh, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
   out[i] = make_float4(p.x, p.y, p.z, 0.0f);
   if (LAST) {
     const float4 x = d.posB[i];

@@ -112,6 +112,20 @@ class Particles:
         check(lib().pbf_estimate_densities(self._ctx, _fptr(rho)), self._ctx)
         return rho
 
+    def set_mesh(self, pos=None, nrm=None, tri=None, skin: float = 0.0) -> None:
+        """Triangle-mesh collider (vertex positions, vertex normals facing the
+        fluid, vertex-index triples, skin clearance) in place of the box
+        walls; None removes it."""
+        if pos is None:
+            check(lib().pbf_set_mesh(self._ctx, None, None, 0, None, 0, 0.0), self._ctx)
+            return
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+        check(lib().pbf_set_mesh(self._ctx, _fptr(pos), _fptr(nrm), len(pos),
+                                 tri.ctypes.data_as(C.POINTER(C.c_int32)), len(tri),
+                                 float(skin)), self._ctx)
+
     def save_snapshot(self, path: str) -> None:
         """Binary checkpoint of the committed state (x, v, rho, simulate_time)."""
         check(lib().pbf_save_snapshot(self._ctx, os.fsencode(path)), self._ctx)

@@ -100,6 +100,25 @@ PBF_API int pbf_step_dt(pbf_ctx *ctx, float dt, int32_t nsteps);
  * (src/particles.cpp:299-301). */
 PBF_API int pbf_step(pbf_ctx *ctx, int32_t nsteps);
 
+/* Triangle-mesh collider, in place of the box's mesh walls: the reference's
+ * scene BVH that clamp/clamp_response ray-cast against (src/particles.cpp:
+ * 51-132, Particles::bvh set by PathTracer::build_accel, src/pathtracer.cpp:
+ * 266; traversal src/bvh.cpp:142-192; triangle test src/static_scene/
+ * triangle.cpp:21-80).  pos, nrm: nv*3 vertex positions and normals (the
+ * mesh's vertex normals, interpolated w n1 + u n2 + v n3 at the hit as the
+ * reference does); tri: nt*3 vertex indices.  With a mesh, a displacement
+ * stops at the nearest triangle hit with 0 <= t <= l (ties: lowest triangle
+ * index) at t - EPS_D, then steps `skin` off the face along its normal,
+ * and prediction slides once along it; the stop planes of wall_stop_mask
+ * and the final per-axis clamp to the box stay.  skin >= 0: the reference
+ * uses EPS_D alone (1e-11, meaningful in FP64); in FP32 a skin well above
+ * the rounding of the coordinates (e.g. 2^-17 of the scene size) keeps
+ * particles from ending on or behind a face and passing through it on the
+ * next move; skin 0 keeps the reference's formula.  nt = 0 removes the
+ * mesh (back to the box walls).  Single-scene contexts only. */
+PBF_API int pbf_set_mesh(pbf_ctx *ctx, const float *pos, const float *nrm, int64_t nv,
+                         const int32_t *tri, int64_t nt, float skin);
+
 /* Execution schedules (no effect on the arithmetic: every schedule runs the
  * same per-particle code in the same order and gives the same bits).
  * No counterpart in the reference (batched extension). */

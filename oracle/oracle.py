@@ -48,7 +48,29 @@ class OrcParams(C.Structure):
         ("poly6_coef", C.c_double),
         ("spiky_coef", C.c_double),
         ("eps_d", C.c_double),
+        ("mesh_pos", C.POINTER(C.c_double)),
+        ("mesh_nrm", C.POINTER(C.c_double)),
+        ("mesh_tri", C.POINTER(C.c_int32)),
+        ("mesh_nt", C.c_int64),
+        ("mesh_skin", C.c_double),
     ]
+
+
+_mesh = None  # (pos, nrm, tri) of the mesh collider, or None
+
+
+def set_mesh(pos=None, nrm=None, tri=None, skin=0.0):
+    """Mesh collider used by every following call (None: the box walls).
+    Positions, normals and the skin are rounded to float32 first, as the
+    GPU path receives them."""
+    global _mesh
+    if pos is None:
+        _mesh = None
+        return
+    pos = np.ascontiguousarray(np.asarray(pos, np.float32), np.float64).reshape(-1, 3)
+    nrm = np.ascontiguousarray(np.asarray(nrm, np.float32), np.float64).reshape(-1, 3)
+    tri = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+    _mesh = (pos, nrm, tri, float(np.float32(skin)))
 
 
 _lib = None
@@ -93,6 +115,14 @@ def params(p, single: bool = True) -> OrcParams:
             setattr(o, name, int(v))
         else:
             setattr(o, name, conv(v))
+    if _mesh is not None:
+        pos, nrm, tri, skin = _mesh
+        o.mesh_skin = skin
+        o.mesh_pos = pos.ctypes.data_as(C.POINTER(C.c_double))
+        o.mesh_nrm = nrm.ctypes.data_as(C.POINTER(C.c_double))
+        o.mesh_tri = tri.ctypes.data_as(C.POINTER(C.c_int32))
+        o.mesh_nt = len(tri)
+        o._keep = _mesh
     return o
 
 

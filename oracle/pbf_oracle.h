@@ -51,6 +51,14 @@ typedef struct orc_params {
   double poly6_coef;    /* 315/(64 pi) as the reference literal (:140) */
   double spiky_coef;    /* 15/pi as the reference literal (:148) */
   double eps_d;         /* EPS_D (CGL/include/CGL/misc.h:11) */
+  /* Triangle-mesh collider in place of the box's mesh walls (mesh_nt 0:
+     the box walls).  Vertex positions and normals (nv*3) and vertex
+     indices (nt*3); converted to the working precision where used. */
+  const double *mesh_pos;
+  const double *mesh_nrm;
+  const int32_t *mesh_tri;
+  int64_t mesh_nt;
+  double mesh_skin;      /* clearance a mesh hit leaves along the face normal */
 } orc_params;
 
 /* Derived constants in working precision, in the order they are computed. */

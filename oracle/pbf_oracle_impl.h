@@ -19,6 +19,7 @@ typedef struct {
   int n_tensile;
   int iters;
   unsigned stop;
+  const orc_params *P; /* for the mesh collider */
 } DER;
 
 /* poly6_kernel, src/particles.cpp:134-141:
@@ -55,6 +56,7 @@ static inline REAL FN(ipow_)(REAL b, int n) {
 }
 
 static void FN(der_init_)(const orc_params *p, REAL dt, DER *d) {
+  d->P = p;
   d->h = (REAL)p->h;
   d->h2 = d->h * d->h;
   d->h6 = FN(ipow_)(d->h, 6);
@@ -128,6 +130,128 @@ void FN(orc_spiky_)(const orc_params *p, const REAL *r, REAL *g) {
 static inline REAL FN(smin_)(REAL a, REAL b) { return (b < a) ? b : a; }
 static inline REAL FN(smax_)(REAL a, REAL b) { return (a < b) ? b : a; }
 
+static inline REAL FN(dot3_)(const REAL *u, const REAL *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+
+static inline void FN(cross3_)(const REAL *u, const REAL *v, REAL *w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+/* Nearest hit of the segment o + t dir, 0 <= t <= tmax, with the mesh:
+ * Triangle::intersect (src/static_scene/triangle.cpp:52-80) on every
+ * triangle in ascending index, keeping the smallest t and, on equal t, the
+ * lowest index (the result of any traversal order of a BVH, cf.
+ * src/bvh.cpp:165-192).  n = w n1 + u n2 + v n3.  Returns the index or -1. */
+static int FN(mesh_nearest_)(const DER *d, const REAL *o, const REAL *dir, REAL tmax, REAL *t_out,
+                            REAL *n_out, REAL *nf_out) {
+  const orc_params *P = d->P;
+  REAL bt = tmax;
+  int best = -1;
+  for (int64_t k = 0; k < P->mesh_nt; k++) {
+    const int32_t *tv = P->mesh_tri + 3 * k;
+    REAL p1[3], e1[3], e2[3], sv[3], s1[3], s2[3];
+    for (int a = 0; a < 3; a++) {
+      p1[a] = (REAL)P->mesh_pos[3 * (size_t)tv[0] + a];
+      e1[a] = (REAL)P->mesh_pos[3 * (size_t)tv[1] + a] - p1[a];
+      e2[a] = (REAL)P->mesh_pos[3 * (size_t)tv[2] + a] - p1[a];
+      sv[a] = o[a] - p1[a];
+    }
+    FN(cross3_)(dir, e2, s1);
+    FN(cross3_)(sv, e1, s2);
+    REAL det = FN(dot3_)(s1, e1);
+    if (det == (REAL)0) continue;
+    REAL t = FN(dot3_)(s2, e2) / det;
+    if (t < (REAL)0 || t > bt || (t == bt && best >= 0)) continue;
+    REAL u = FN(dot3_)(s1, sv) / det;
+    REAL v = FN(dot3_)(s2, dir) / det;
+    REAL w = ((REAL)1 - u) - v;
+    if (u < (REAL)0 || u > (REAL)1 || v < (REAL)0 || v > (REAL)1 || w < (REAL)0 || w > (REAL)1)
+      continue;
+    bt = t;
+    best = (int)k;
+    FN(cross3_)(e1, e2, nf_out);
+    for (int a = 0; a < 3; a++)
+      n_out[a] = (w * (REAL)P->mesh_nrm[3 * (size_t)tv[0] + a] +
+                  u * (REAL)P->mesh_nrm[3 * (size_t)tv[1] + a]) +
+                 v * (REAL)P->mesh_nrm[3 * (size_t)tv[2] + a];
+  }
+  *t_out = bt;
+  return best;
+}
+
+/* Analytic stop planes (strict 0 < t < l), src/particles.cpp:59-74. */
+static int FN(stop_planes_)(const DER *d, const REAL *p, const REAL *dir, REAL *l) {
+  int intersect = 0;
+  for (int a = 2; a >= 0; a--) {
+    if (dir[a] == (REAL)0) continue;
+    int hw = dir[a] > (REAL)0;
+    if (!((d->stop >> (2 * a + hw)) & 1u)) continue;
+    REAL plane = hw ? d->hi[a] : d->lo[a];
+    REAL pt = (plane - p[a]) / dir[a];
+    if (pt > (REAL)0 && pt < *l) {
+      *l = pt;
+      intersect = 1;
+    }
+  }
+  return intersect;
+}
+
+/* After a hit on a face with normal nf, step off it by the skin along the
+ * unit normal, on the side the segment came from (EPS_D alone is below float
+ * resolution; see the HIP collider). */
+static void FN(push_off_)(REAL *p, const REAL *dir, const REAL *nf, REAL skin) {
+  if (!(skin > (REAL)0)) return;
+  REAL dn = FN(dot3_)(dir, nf);
+  REAL k = skin / SQRT(FN(dot3_)(nf, nf));
+  REAL s = dn > (REAL)0 ? -k : k;
+  p[0] = p[0] + s * nf[0];
+  p[1] = p[1] + s * nf[1];
+  p[2] = p[2] + s * nf[2];
+}
+
+/* clamp / clamp_response against the mesh (src/particles.cpp:51-132): stop
+ * planes, nearest triangle, move to t - EPS_D; in prediction slide once
+ * along (D - (D.n) n).unit() for half the remaining length against the
+ * mesh only; then the per-axis clamp.  A zero tangent does not slide. */
+static void FN(collide_mesh_)(const DER *d, REAL *p, const REAL *D, const REAL *dir, REAL l,
+                              REAL total, int slide) {
+  const REAL skin = (REAL)d->P->mesh_skin;
+  int intersect = FN(stop_planes_)(d, p, dir, &l);
+  REAL th, n[3] = {0, 0, 0}, nf[3] = {0, 0, 0};
+  int hit = FN(mesh_nearest_)(d, p, dir, l, &th, n, nf);
+  if (hit >= 0) l = th;
+  if (hit >= 0 || intersect) {
+    REAL s = l - d->eps_d;
+    p[0] = p[0] + s * dir[0];
+    p[1] = p[1] + s * dir[1];
+    p[2] = p[2] + s * dir[2];
+    if (hit >= 0) FN(push_off_)(p, dir, nf, skin);
+    if (slide && hit >= 0 && !intersect && FN(dot3_)(dir, n) > (REAL)-1) {
+      REAL dn = FN(dot3_)(D, n);
+      REAL T[3] = {D[0] - dn * n[0], D[1] - dn * n[1], D[2] - dn * n[2]};
+      REAL tl2 = T[0] * T[0] + T[1] * T[1] + T[2] * T[2];
+      if (tl2 > (REAL)0) {
+        REAL rn = (REAL)1 / SQRT(tl2);
+        REAL d2[3] = {rn * T[0], rn * T[1], rn * T[2]};
+        REAL l2 = (total - l) * (REAL)0.5, t2, n2[3], nf2[3];
+        int hit2 = FN(mesh_nearest_)(d, p, d2, l2, &t2, n2, nf2);
+        if (hit2 >= 0) l2 = t2;
+        REAL s2 = l2 - d->eps_d;
+        p[0] = p[0] + s2 * d2[0];
+        p[1] = p[1] + s2 * d2[1];
+        p[2] = p[2] + s2 * d2[2];
+        if (hit2 >= 0) FN(push_off_)(p, d2, nf2, skin);
+      }
+    }
+  } else {
+    p[0] = p[0] + D[0];
+    p[1] = p[1] + D[1];
+    p[2] = p[2] + D[2];
+  }
+  for (int a = 0; a < 3; a++) p[a] = FN(smax_)(d->lo_e[a], FN(smin_)(d->hi_e[a], p[a]));
+}
+
 /* Ray clip against the collision box.  Restates the shared body of clamp
  * (src/particles.cpp:51-84) and clamp_response (:87-132) for an axis-aligned
  * box [lo, hi] whose walls are either analytic stop planes (the reference's
@@ -138,6 +262,10 @@ static inline REAL FN(smax_)(REAL a, REAL b) { return (a < b) ? b : a; }
  * per-axis clamp to [lo + EPS_D, hi - EPS_D] (:81-83, :129-131). */
 static void FN(collide_)(const DER *d, REAL *p, const REAL *D, const REAL *dir, REAL l,
                          REAL total, int slide) {
+  if (d->P && d->P->mesh_nt > 0) {
+    FN(collide_mesh_)(d, p, D, dir, l, total, slide);
+    return;
+  }
   int intersect = 0;
   for (int a = 2; a >= 0; a--) {
     if (dir[a] == (REAL)0) continue;

@@ -424,3 +424,128 @@ def test_density_field_single_particle_closed_form(orc):
     got = orc.density_at(p, x, q, orc.NBR_GRID, np.float64)
     want = np.where(r < 0.1, P.POLY6_LITERAL * (0.01 - r * r) ** 3 / 0.1 ** 9, 0.0)
     assert np.allclose(got, want, rtol=1e-9, atol=1e-12)
+
+
+# ------------------------------------------------------------ mesh collider --
+@pytest.fixture
+def no_mesh(orc):
+    yield orc
+    orc.set_mesh(None)
+
+
+@pytest.mark.parametrize("stop_mask", [0, (1 << 3) | (1 << 5)])
+def test_mesh_box_equals_analytic_box_walls(no_mesh, stop_mask):
+    """The box's walls as 12 triangles with inward normals (faces with a
+    stop plane left out) collide like the analytic box walls, in double:
+    ties the triangle test, nearest-hit search, interpolated normal and
+    slide to the separately pinned box collider (catches a wrong sign in the
+    barycentric test, a normal interpolated with permuted weights, or a
+    slide along the wrong tangent)."""
+    from fluid_amd import mesh
+
+    orc = no_mesh
+    # bounds exact in float32 (the mesh is handed over in float32)
+    lo, hi = (-1.0, 0.0, -1.0), (1.0, 1.5, 1.0)
+    p = P.scene_params(0.1, 0.05, lo, hi, dt=0.01).replace(wall_stop_mask=stop_mask)
+    rng = np.random.default_rng(21)
+    pos = rng.uniform(np.array(lo) + 0.02, np.array(hi) - 0.02, size=(400, 3))
+    vel = rng.normal(0, 60.0, size=(400, 3))  # long moves: many reach a wall
+    skip = [w for w in range(6) if (stop_mask >> w) & 1]
+    box_mesh = mesh.box(lo, hi, skip)
+    moved = 0
+    for i in range(len(pos)):
+        a = orc.clamp_response(p, pos[i], vel[i], 0.01, np.float64)
+        ca = orc.clamp(p, pos[i], vel[i] * 0.01, np.float64)
+        orc.set_mesh(*box_mesh)
+        b = orc.clamp_response(p, pos[i], vel[i], 0.01, np.float64)
+        cb = orc.clamp(p, pos[i], vel[i] * 0.01, np.float64)
+        orc.set_mesh(None)
+        assert np.allclose(a, b, rtol=0, atol=1e-9), (i, a, b)
+        assert np.allclose(ca, cb, rtol=0, atol=1e-9), (i, ca, cb)
+        moved += not np.allclose(a, pos[i] + vel[i] * 0.01)
+    assert moved > 150
+
+
+def test_mesh_ray_triangle_against_plane_solve(no_mesh):
+    """Segment vs one triangle: hit and t agree with an independent solve
+    (plane intersection, then barycentric coordinates by least squares),
+    away from the triangle's edges."""
+    orc = no_mesh
+    p = P.scene_params(0.1, 0.05, (-10, -10, -10), (10, 10, 10), dt=0.01)
+    rng = np.random.default_rng(5)
+    checked = hits = 0
+    for _ in range(600):
+        tri_p = rng.uniform(-1, 1, size=(3, 3)).astype(np.float32).astype(np.float64)
+        nrm = np.tile(rng.normal(size=3), (3, 1))
+        o = rng.uniform(-1.5, 1.5, size=3)
+        # aim at a point of the triangle's plane (inside or outside it),
+        # reached at a random fraction of the segment (beyond its end or not)
+        bc = rng.uniform(-0.3, 1.0, size=3)
+        if bc.sum() < 0.5:
+            continue
+        target = (bc / bc.sum()) @ tri_p
+        dp = (target - o) / rng.uniform(0.3, 1.6)
+        e1, e2 = tri_p[1] - tri_p[0], tri_p[2] - tri_p[0]
+        n = np.cross(e1, e2)
+        dn = n @ dp
+        want = None
+        if abs(dn) > 1e-6:
+            s = (n @ (tri_p[0] - o)) / dn  # fraction of dp
+            q = o + s * dp
+            uv = np.linalg.lstsq(np.stack([e1, e2], 1), q - tri_p[0], rcond=None)[0]
+            bary = np.array([1 - uv.sum(), uv[0], uv[1]])
+            if np.all(np.abs(bary) > 1e-6) and np.all(np.abs(bary - 1) > 1e-6) and abs(s) > 1e-6 \
+                    and abs(s - 1) > 1e-6:
+                want = s if (0 < s < 1 and np.all(bary > 0)) else None
+            else:
+                continue
+        else:
+            continue
+        orc.set_mesh(tri_p, nrm, [[0, 1, 2]])
+        got = orc.clamp(p, o, dp, np.float64)
+        orc.set_mesh(None)
+        l = np.linalg.norm(dp)
+        if want is None:
+            assert np.allclose(got, o + dp, atol=1e-12)
+        else:
+            hits += 1
+            assert np.allclose(got, o + (want * l - P.EPS_D) * dp / l, atol=1e-9)
+        checked += 1
+    assert checked > 400 and hits > 100
+
+
+def test_mesh_sphere_obstacle_stops_and_slides(no_mesh):
+    """A particle falling onto the top of a sphere mesh stops on it and, in
+    prediction, slides sideways; the position correction (clamp) never
+    slides; nothing ends inside the polyhedron."""
+    from fluid_amd import mesh
+
+    orc = no_mesh
+    c, r = np.array([0.0, 0.5, 0.0]), 0.3
+    sph = tuple(np.asarray(a, np.float32).astype(np.float64) if a.dtype.kind == "f" else a
+                for a in mesh.icosphere(c, r, 2))
+    p = P.scene_params(0.1, 0.05, (-1, 0, -1), (1, 1.49, 1), dt=0.01)
+    orc.set_mesh(*sph)
+    pos = np.array([0.05, 0.85, 0.0])  # above the top, off-centre
+    v = np.array([0.0, -20.0, 0.0])      # 0.2 m in one step: reaches the sphere
+    a = orc.clamp_response(p, pos, v, 0.01, np.float64)
+    b = orc.clamp(p, pos, v * 0.01, np.float64)
+    assert b[0] == pos[0] and b[2] == pos[2] and b[1] < pos[1]   # straight down, stopped
+    assert a[0] > pos[0]                                         # slid away from the axis
+    pos_, nrm_, tri_ = sph
+    # inside test: on the inner side of every face plane of the convex hull
+    def inside(q):
+        for t in tri_:
+            n = np.cross(pos_[t[1]] - pos_[t[0]], pos_[t[2]] - pos_[t[0]])
+            n = n if n @ (pos_[t[0]] - c) > 0 else -n
+            if n @ (q - pos_[t[0]]) > -1e-9 * np.linalg.norm(n):  # on or outside this face
+                return False
+        return True
+    assert not inside(a) and not inside(b)
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        d = rng.normal(size=3)
+        q = c + d / np.linalg.norm(d) * (r + 0.05)
+        vv = (c - q) * rng.uniform(5, 30)
+        assert not inside(orc.clamp_response(p, q, vv, 0.01, np.float64))
+        assert not inside(orc.clamp(p, q, vv * 0.01, np.float64))
