@@ -236,9 +236,10 @@ __device__ __forceinline__ uint32_t cell_lin(const SceneGrid& g, float3 p, int* 
 
 // applyForceVelocity (src/particles.cpp:175-183) + cell hash + histogram.
 // With MOVE=false x* = x (used by the load-time density pass).
+// Prediction and cell of particle i; returns the cell.
 template <bool MOVE>
-__device__ __forceinline__ void predict_one(uint32_t i, const Dev& d, const SimConst& c,
-                                            const SceneGrid& g) {
+__device__ __forceinline__ uint32_t predict_cell(uint32_t i, const Dev& d, const SimConst& c,
+                                                 const SceneGrid& g) {
   const float4 x = d.posA[i];
   float3 p = make_float3(x.x, x.y, x.z);
   if (MOVE) {
@@ -250,7 +251,36 @@ __device__ __forceinline__ void predict_one(uint32_t i, const Dev& d, const SimC
   d.predP[i] = make_float4(p.x, p.y, p.z, 0.0f);
   const uint32_t cl = cell_lin(g, p);
   d.cell[i] = cl;
+  return cl;
+}
+
+template <bool MOVE>
+__device__ __forceinline__ void predict_one(uint32_t i, const Dev& d, const SimConst& c,
+                                            const SceneGrid& g) {
+  const uint32_t cl = predict_cell<MOVE>(i, d, c, g);
   d.arrival[i] = atomicAdd(&d.cellCount[cl], 1u);
+}
+
+// Histogram slot of each lane's cell with one atomic per run of equal cells
+// in consecutive lanes (the state is sorted by the previous step's cells,
+// so a wave holds a few runs of ~8).  Every lane of the wave must call it;
+// lanes with valid = false take no slot.  The slots of one cell are still
+// 0..count-1; their order within the cell is irrelevant (the sort ranks by
+// id).
+__device__ __forceinline__ uint32_t wave_run_slot(uint32_t cl, bool valid,
+                                                  uint32_t* __restrict__ count) {
+  const uint32_t lane = __lane_id();
+  const uint32_t prev = __shfl_up(cl, 1);
+  const bool head = lane == 0 || prev != cl;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all for lane 63)
+  const uint32_t start = 63u - (uint32_t)__clzll((long long)(heads & upto));
+  const unsigned long long after = heads & ~upto;
+  const uint32_t end = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+  uint32_t base = 0;
+  if (head && valid) base = atomicAdd(count + cl, end - start);
+  base = __shfl(base, (int)start);
+  return base + (lane - start);
 }
 
 __device__ __forceinline__ void scatter_one(uint32_t i, const Dev& d) {
@@ -604,7 +634,11 @@ __device__ __forceinline__ uint32_t gather_index() {
 template <bool MULTI, bool MOVE>
 __global__ __launch_bounds__(kBlock) void k_predict(Dev d, SimConst c, SceneGrid g0) {
   const uint32_t i = flat_index();
-  if (i < d.n) predict_one<MOVE>(i, d, c, grid_of<MULTI>(i, g0, d));
+  const bool valid = i < d.n;
+  uint32_t cl = 0xffffffffu;
+  if (valid) cl = predict_cell<MOVE>(i, d, c, grid_of<MULTI>(i, g0, d));
+  const uint32_t slot = wave_run_slot(cl, valid, d.cellCount);
+  if (valid) d.arrival[i] = slot;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(Dev d) {
