@@ -82,13 +82,24 @@ def cpu_baseline(sc, steps: int):
             model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
+    # The same restatement on one core, one step.
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    oracle.step(sc.params, sc.x, sc.v, 1)
+    dt1 = time.perf_counter() - t0
+    oracle.set_threads(threads)
+    log(f"cpu baseline: {sc.n * steps / dt:.3e} particle-steps/s on {threads} threads, "
+        f"{sc.n / dt1:.3e} on 1")
     return {
         "value": sc.n * steps / dt,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{sc.name}: {steps} steps from the initial state, oracle/ restatement "
-                  f"(FP32, uniform grid, Jacobi XSPH, OpenMP), {dt:.2f} s; host {model}",
+                  f"(FP32, uniform grid, Jacobi XSPH, OpenMP), {dt:.2f} s; host {model}, "
+                  f"nproc {os.cpu_count()}, {threads} usable",
+        "one_core": {"value": sc.n / dt1, "unit": "particle-steps/s", "cores": 1,
+                     "sample": f"{sc.name}: 1 step, {dt1:.2f} s"},
     }
 
 
