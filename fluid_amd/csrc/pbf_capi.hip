@@ -265,13 +265,20 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr) {
   x->dbg_pred = d.predQ;
   x->dbg_vel = d.velB;
   if (stage == PBF_STAGE_PREDICT) return PBF_OK;
-  PASS(PBF_PASS_NEIGHBORS, pbf::launch_nbr_build(*x, c));
-  x->has_nbrs = true;
-  if (stage == PBF_STAGE_NEIGHBORS) return PBF_OK;
   float4* cur = d.predQ;
   float4* other = d.predP;
+#ifndef PBF_FUSE_NBR_LAMBDA
+#define PBF_FUSE_NBR_LAMBDA 1
+#endif
+  const bool fuse = PBF_FUSE_NBR_LAMBDA && stage != PBF_STAGE_NEIGHBORS;
+  if (fuse)
+    PASS(PBF_PASS_NEIGHBORS, pbf::launch_nbr_lambda(*x, c, other));
+  else
+    PASS(PBF_PASS_NEIGHBORS, pbf::launch_nbr_build(*x, c));
+  x->has_nbrs = true;
+  if (stage == PBF_STAGE_NEIGHBORS) return PBF_OK;
   for (int it = 0; it < x->p.iterations; it++) {
-    PASS(PBF_PASS_LAMBDA, pbf::launch_lambda(*x, c, cur, other));
+    if (!(fuse && it == 0)) PASS(PBF_PASS_LAMBDA, pbf::launch_lambda(*x, c, cur, other));
     if (stage == PBF_STAGE_LAMBDA0 && it == 0) {
       x->dbg_pred = other;
       return PBF_OK;

@@ -105,6 +105,8 @@ hipError_t launch_scan(Ctx& x);
 hipError_t launch_scatter_rank(Ctx& x);
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c);
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out);
+// Neighbour build and the first λ pass (in = predQ) in one launch.
+hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out);
 hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last);
 hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out);
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in);

@@ -657,6 +657,19 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(Dev d, float h2, SceneGrid
   if (i < d.n) nbr_one(i, d, h2, grid_of<MULTI>(i, g0, d));
 }
 
+// The neighbour build followed, in the same thread, by the first density/λ
+// pass over the list it has just written (its own stores, read back in
+// program order).  Waves in the memory-bound build overlap waves in the
+// VALU-bound λ pass.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneGrid g0,
+                                                       float4* out) {
+  const uint32_t i = gather_index();
+  if (i >= d.n) return;
+  nbr_one(i, d, c.h2, grid_of<MULTI>(i, g0, d));
+  lambda_one(i, d, c, d.predQ, out);
+}
+
 __global__ __launch_bounds__(kBlock) void k_lambda(Dev d, SimConst c, const float4* in,
                                                    float4* out) {
   const uint32_t i = gather_index();
@@ -901,6 +914,15 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
     k_nbr_build<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
   else
     k_nbr_build<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
+  return hipGetLastError();
+}
+
+hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
+  const Dev d = dev_of(x);
+  if (multi(x))
+    k_nbr_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
+  else
+    k_nbr_lambda<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
   return hipGetLastError();
 }
 
