@@ -435,13 +435,17 @@ int forward_step(pbf_adj* a, const pbf::SimConst& k, bool stash) {
   AHIP(a, pbf::launch_nbr_build(*c, k));
   if (stash) pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, a->stash);
   for (int it = 0; it < c->p.iterations; it++) {
+    const bool last = it == c->p.iterations - 1;
     AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP));
-    AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, it == c->p.iterations - 1));
+    AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, last));
+    // The last iteration writes x* and v as XSPH records; the adjoint reads
+    // them from predQ and velB.
+    if (last) AHIP(a, pbf::launch_unpack_record(*c, d.predQ, d.velB));
     if (stash)
       pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ,
                                                            a->stash + (size_t)(it + 1) * np);
   }
-  AHIP(a, pbf::launch_xsph(*c, k, d.predQ, d.predP));
+  AHIP(a, pbf::launch_xsph(*c, k, d.predP));
   if (!stash) AHIP(a, pbf::launch_vort_commit(*c, k, d.predP));
   AHIP(a, hipGetLastError());
   return PBF_OK;

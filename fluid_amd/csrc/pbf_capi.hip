@@ -283,15 +283,17 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr) {
       x->dbg_pred = other;
       return PBF_OK;
     }
-    PASS(PBF_PASS_DELTAP, pbf::launch_deltap(*x, c, other, cur, it == x->p.iterations - 1));
-    if (stage == PBF_STAGE_DELTAP0 && it == 0) {
+    const bool last = it == x->p.iterations - 1;
+    PASS(PBF_PASS_DELTAP, pbf::launch_deltap(*x, c, other, cur, last));
+    // The last iteration leaves x* and v only in the XSPH records; a debug
+    // stop unpacks them where the earlier stages keep them.
+    if ((stage == PBF_STAGE_DELTAP0 && it == 0) || (stage == PBF_STAGE_SOLVE && last)) {
+      if (last) HIPCHK(x, pbf::launch_unpack_record(*x, cur, d.velB));
       x->dbg_pred = cur;
       return PBF_OK;
     }
   }
-  x->dbg_pred = cur;
-  if (stage == PBF_STAGE_SOLVE) return PBF_OK;
-  PASS(PBF_PASS_XSPH, pbf::launch_xsph(*x, c, cur, other));
+  PASS(PBF_PASS_XSPH, pbf::launch_xsph(*x, c, other));
   x->dbg_pred = other;
   if (stage == PBF_STAGE_XSPH) return PBF_OK;
   PASS(PBF_PASS_VORTICITY, pbf::launch_vort_commit(*x, c, other));

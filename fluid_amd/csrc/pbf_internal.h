@@ -108,7 +108,10 @@ hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* ou
 // Neighbour build and the first λ pass (in = predQ) in one launch.
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out);
 hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last);
-hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out);
+// XSPH reads x* and v from the records the last delta-p wrote.
+hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out);
+// x* and v of the last iteration's records into float4 arrays.
+hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel);
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in);
 hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int first, int count,
                             float* dst);

@@ -34,11 +34,6 @@ constexpr int kBlock = 256;
 #ifndef PBF_UNROLL_XSPH
 #define PBF_UNROLL_XSPH 8
 #endif
-// XSPH gathers x* and v of a neighbour from one 32-byte record written by
-// the last delta-p pass (one cache line per neighbour instead of two).
-#ifndef PBF_XV
-#define PBF_XV 1
-#endif
 constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
 constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers two float4 per neighbour
 
@@ -458,6 +453,12 @@ __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimCo
   out[i] = make_float4(xi.x, xi.y, xi.z, lambda);
 }
 
+// (x*, v) record of sorted particle i after the solve: 6 consecutive floats,
+// so XSPH gathers both with two adjacent 12-byte loads.
+__device__ __forceinline__ float3* xv_record(const Dev& d, uint32_t i) {
+  return reinterpret_cast<float3*>(d.xv) + 2 * (size_t)i;
+}
+
 // newtonStepUpdatePosition (src/particles.cpp:206-213) with s_corr (:195)
 // and clamp (:51-84).  in = (x*, lambda), out = (x*', 0).  The last
 // iteration also forms v = (x* - x) / dt (updateVelocity, :215-217).
@@ -486,16 +487,16 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   dz = dz * c.inv_rho0;
   const float3 p =
       clamp_dp(g, c, d.mesh, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
-  out[i] = make_float4(p.x, p.y, p.z, 0.0f);
-  if (LAST) {
+  if (!LAST) {
+    out[i] = make_float4(p.x, p.y, p.z, 0.0f);
+  } else {
+    // The last iteration writes only the (x*, v) record that XSPH gathers.
     const float4 x = d.posB[i];
-    const float4 v = make_float4(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y),
-                                 c.inv_dt * (p.z - x.z), 0.0f);
-    d.velB[i] = v;
-#if PBF_XV
-    d.xv[2 * (size_t)i] = make_float4(p.x, p.y, p.z, 0.0f);
-    d.xv[2 * (size_t)i + 1] = v;
-#endif
+    const float3 v = make_float3(c.inv_dt * (p.x - x.x), c.inv_dt * (p.y - x.y),
+                                 c.inv_dt * (p.z - x.z));
+    float3* r = xv_record(d, i);
+    r[0] = make_float3(p.x, p.y, p.z);
+    r[1] = v;
   }
 }
 
@@ -503,26 +504,16 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
 // every neighbour velocity is the pre-XSPH v.  out = (x*, |omega|),
 // vxsph = (v + C*visc, density), vort = (omega, 0).
 __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimConst& c,
-                                         const float4* __restrict__ in,
                                          float4* __restrict__ out) {
-  const float4* __restrict__ vel = d.velB;
-  const float4* __restrict__ xv = d.xv;  // (x*, v) of particle j at 2j, 2j+1
-  const float4 xi = in[i];
-  const float4 vi = vel[i];
+  const float3* __restrict__ xv = reinterpret_cast<const float3*>(d.xv);
+  const float3 xi = xv[2 * (size_t)i], vi = xv[2 * (size_t)i + 1];
   const int cnt = (int)d.ncnt[i];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f, sx = 0.0f, sy = 0.0f, sz = 0.0f, dens = 0.0f;
   struct XV {
-    float4 x, v;
+    float3 x, v;
   };
   for_each_neighbor<kUnrollXsph>(
-      i, cnt, d.K4, d.nbr,
-      [&](uint32_t j) {
-#if PBF_XV
-        return XV{at(xv, 2 * j), at(xv, 2 * j + 1)};
-#else
-        return XV{at(in, j), at(vel, j)};
-#endif
-      },
+      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
       [&](uint32_t, const XV& q) {
         const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
         const float rx = xi.x - q.x.x, ry = xi.y - q.x.y, rz = xi.z - q.x.z;
@@ -683,10 +674,21 @@ __global__ __launch_bounds__(kBlock) void k_deltap(Dev d, SimConst c, SceneGrid 
   if (i < d.n) deltap_one<LAST>(i, d, c, grid_of<MULTI>(i, g0, d), in, out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_xsph(Dev d, SimConst c, const float4* in,
-                                                 float4* out) {
+__global__ __launch_bounds__(kBlock) void k_xsph(Dev d, SimConst c, float4* out) {
   const uint32_t i = gather_index();
-  if (i < d.n) xsph_one(i, d, c, in, out);
+  if (i < d.n) xsph_one(i, d, c, out);
+}
+
+// Copies the (x*, v) records of the last iteration into float4 arrays (for
+// debug stops and the adjoint, which read x* and v there).
+__global__ __launch_bounds__(kBlock) void k_unpack_record(Dev d, float4* __restrict__ pos,
+                                                          float4* __restrict__ vel) {
+  const uint32_t i = flat_index();
+  if (i >= d.n) return;
+  const float3* r = xv_record(d, i);
+  const float3 x = r[0], v = r[1];
+  pos[i] = make_float4(x.x, x.y, x.z, 0.0f);
+  vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vort_commit(Dev d, SimConst c, const float4* in) {
@@ -769,7 +771,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const S
         }
         phase_barrier();
       }
-      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) xsph_one(i, d, c, d.predQ, d.predP);
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) xsph_one(i, d, c, d.predP);
       phase_barrier();
       for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) vort_one(i, d, c, d.predP);
       phase_barrier();
@@ -948,8 +950,13 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
   return hipGetLastError();
 }
 
-hipError_t launch_xsph(Ctx& x, const SimConst& c, const float4* in, float4* out) {
-  k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
+  k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel) {
+  k_unpack_record<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), pos, vel);
   return hipGetLastError();
 }
 
