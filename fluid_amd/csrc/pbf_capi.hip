@@ -257,11 +257,16 @@ int mark(Ctx* x, PassEvents* pe, int pass, bool begin) {
   } while (0)
 
 // One step, optionally stopping after a debug stage (0 = full step).
-int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr) {
+// predicted: the previous step of the same call already ran this step's
+// prediction; predict_next: fuse the next step's prediction into this
+// step's last pass (both steps of one call, same dt).
+int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr,
+             bool predicted = false, bool predict_next = false) {
   pbf::DevState& d = x->d;
-  PASS(PBF_PASS_PREDICT, pbf::launch_predict(*x, c));
+  if (!predicted) PASS(PBF_PASS_PREDICT, pbf::launch_predict(*x, c));
   PASS(PBF_PASS_SCAN, pbf::launch_scan(*x));
-  PASS(PBF_PASS_SORT, pbf::launch_scatter_rank(*x));
+  // A prediction fused into the previous step's last pass sits in vort.
+  PASS(PBF_PASS_SORT, pbf::launch_scatter_rank(*x, predicted ? d.vort : nullptr));
   x->dbg_pred = d.predQ;
   x->dbg_vel = d.velB;
   if (stage == PBF_STAGE_PREDICT) return PBF_OK;
@@ -296,7 +301,10 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr) {
   PASS(PBF_PASS_XSPH, pbf::launch_xsph(*x, c, other));
   x->dbg_pred = other;
   if (stage == PBF_STAGE_XSPH) return PBF_OK;
-  PASS(PBF_PASS_VORTICITY, pbf::launch_vort_commit(*x, c, other));
+  if (predict_next)
+    PASS(PBF_PASS_VORTICITY, pbf::launch_vort_commit_predict(*x, c, other));
+  else
+    PASS(PBF_PASS_VORTICITY, pbf::launch_vort_commit(*x, c, other));
   x->dbg_vel = d.velA;
   return PBF_OK;
 }
@@ -616,7 +624,7 @@ int enqueue_steps(Ctx* x, float dt, int nsteps) {
     for (int s = 0; s < nsteps; s++) HIPCHK(x, hipGraphLaunch(x->graph_exec, x->stream));
   } else {
     for (int s = 0; s < nsteps; s++) {
-      const int rc = run_step(x, k, 0);
+      const int rc = run_step(x, k, 0, nullptr, s > 0, s + 1 < nsteps);
       if (rc != PBF_OK) return rc;
     }
   }
@@ -689,7 +697,7 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
   // the end, so the steps run back to back as in pbf_step.
   PassEvents pe;
   for (int s = 0; s < nsteps && rc == PBF_OK; s++) {
-    rc = run_step(x, k, 0, &pe);
+    rc = run_step(x, k, 0, &pe, s > 0, s + 1 < nsteps);
     if (rc == PBF_OK) x->sim_time += (double)dt;
   }
   if (rc == PBF_OK) {

@@ -102,7 +102,9 @@ void derive_grid(const pbf_params& p, const float* lo, const float* hi, SceneGri
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_predict(Ctx& x, const SimConst& c);
 hipError_t launch_scan(Ctx& x);
-hipError_t launch_scatter_rank(Ctx& x);
+// Counting-sort scatter and (cell, id) rank; pred is where the unsorted
+// predictions are (NULL: predP).
+hipError_t launch_scatter_rank(Ctx& x, const float4* pred = nullptr);
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c);
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out);
 // Neighbour build and the first λ pass (in = predQ) in one launch.
@@ -113,6 +115,9 @@ hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out);
 // x* and v of the last iteration's records into float4 arrays.
 hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel);
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in);
+// Vorticity and commit fused with the next step's prediction (zeroes the
+// histogram first); the next step then starts at the scan.
+hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* in);
 hipError_t launch_unpermute(Ctx& x, const float4* ids, const float4* src, int first, int count,
                             float* dst);
 hipError_t launch_unpermute_u32(Ctx& x, const float4* ids, const uint32_t* src, uint32_t* dst);

@@ -231,22 +231,29 @@ __device__ __forceinline__ uint32_t cell_lin(const SceneGrid& g, float3 p, int* 
 
 // applyForceVelocity (src/particles.cpp:175-183) + cell hash + histogram.
 // With MOVE=false x* = x (used by the load-time density pass).
-// Prediction and cell of particle i; returns the cell.
+// Prediction and cell of particle i from its committed x and v; returns
+// the cell.
 template <bool MOVE>
-__device__ __forceinline__ uint32_t predict_cell(uint32_t i, const Dev& d, const SimConst& c,
-                                                 const SceneGrid& g) {
-  const float4 x = d.posA[i];
+__device__ __forceinline__ uint32_t predict_from(uint32_t i, const Dev& d, const SimConst& c,
+                                                 const SceneGrid& g, float4 x, float4 v,
+                                                 float4* __restrict__ pred) {
   float3 p = make_float3(x.x, x.y, x.z);
   if (MOVE) {
-    float4 v = d.velA[i];
     v.y = v.y - c.gdt;
     p = clamp_response(g, c, d.mesh, p, make_float3(v.x, v.y, v.z));
     d.velA[i] = v;
   }
-  d.predP[i] = make_float4(p.x, p.y, p.z, 0.0f);
+  pred[i] = make_float4(p.x, p.y, p.z, 0.0f);
   const uint32_t cl = cell_lin(g, p);
   d.cell[i] = cl;
   return cl;
+}
+
+template <bool MOVE>
+__device__ __forceinline__ uint32_t predict_cell(uint32_t i, const Dev& d, const SimConst& c,
+                                                 const SceneGrid& g) {
+  return predict_from<MOVE>(i, d, c, g, d.posA[i], MOVE ? d.velA[i] : make_float4(0, 0, 0, 0),
+                           d.predP);
 }
 
 template <bool MOVE>
@@ -286,7 +293,8 @@ __device__ __forceinline__ void scatter_one(uint32_t i, const Dev& d) {
 // Rank of a particle inside its cell = number of cell members with a smaller
 // id; the gather target is cellStart + rank, which makes the order (cell, id)
 // independent of atomic arrival order.
-__device__ __forceinline__ void rank_one(uint32_t i, const Dev& d) {
+__device__ __forceinline__ void rank_one(uint32_t i, const Dev& d,
+                                         const float4* __restrict__ pred) {
   const uint32_t c = d.cell[i];
   const uint32_t s = d.cellStart[c];
   const uint32_t e = min(d.cellStart[c + 1], d.n);
@@ -298,7 +306,7 @@ __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d) {
   if (dst >= d.n) return;
   d.posB[dst] = x;
   d.velB[dst] = d.velA[i];
-  d.predQ[dst] = d.predP[i];
+  d.predQ[dst] = pred[i];
   d.cellSorted[dst] = c;
 }
 
@@ -536,8 +544,11 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
 
 // applyVorticity + updatePosition (src/particles.cpp:236-248).
 // in = (x*, |omega|).  Commits posA = (x*, id), velA = (v, density).
+// Returns the committed x (xyz, id) and v of particle i; writes them unless
+// the caller goes on with the next step's prediction (commit = false).
 __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimConst& c,
-                                         const float4* __restrict__ in) {
+                                         const float4* __restrict__ in, float4* x_out = nullptr,
+                                         float4* v_out = nullptr, bool commit = true) {
   const float4 xi = in[i];
   const int cnt = (int)d.ncnt[i];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
@@ -564,8 +575,11 @@ __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimCons
     v.y = v.y + c.dtv * c1;
     v.z = v.z + c.dtv * c2;
   }
-  d.posA[i] = make_float4(xi.x, xi.y, xi.z, d.posB[i].w);
-  d.velA[i] = v;
+  const float4 x = make_float4(xi.x, xi.y, xi.z, d.posB[i].w);
+  d.posA[i] = x;
+  if (commit) d.velA[i] = v;
+  if (x_out) *x_out = x;
+  if (v_out) *v_out = v;
 }
 
 // Density of the committed particles at point p (estimateDensityAt,
@@ -637,9 +651,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Dev d) {
   if (i < d.n) scatter_one(i, d);
 }
 
-__global__ __launch_bounds__(kBlock) void k_rank(Dev d) {
+__global__ __launch_bounds__(kBlock) void k_rank(Dev d, const float4* pred) {
   const uint32_t i = flat_index();
-  if (i < d.n) rank_one(i, d);
+  if (i < d.n) rank_one(i, d, pred);
 }
 
 template <bool MULTI>
@@ -694,6 +708,26 @@ __global__ __launch_bounds__(kBlock) void k_unpack_record(Dev d, float4* __restr
 __global__ __launch_bounds__(kBlock) void k_vort_commit(Dev d, SimConst c, const float4* in) {
   const uint32_t i = gather_index();
   if (i < d.n) vort_one(i, d, c, in);
+}
+
+// Vorticity confinement and commit, then, from the committed x and v still
+// in registers, the next step's prediction, cell and histogram slot (as
+// k_predict; cellCount must be zero).  Used between the steps of one call.
+// The prediction goes to the vort array (this particle's omega, already
+// read): predP is `in`, still being gathered by other threads.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_vort_commit_predict(Dev d, SimConst c, SceneGrid g0,
+                                                                const float4* in) {
+  const uint32_t i = gather_index();
+  const bool valid = i < d.n;
+  uint32_t cl = 0xffffffffu;
+  if (valid) {
+    float4 x, v;
+    vort_one(i, d, c, in, &x, &v, false);
+    cl = predict_from<true>(i, d, c, grid_of<MULTI>(i, g0, d), x, v, d.vort);
+  }
+  const uint32_t slot = wave_run_slot(cl, valid, d.cellCount);
+  if (valid) d.arrival[i] = slot;
 }
 
 template <bool MULTI>
@@ -755,7 +789,7 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const S
       phase_barrier();
       for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) scatter_one(i, d);
       phase_barrier();
-      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) rank_one(i, d);
+      for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) rank_one(i, d, d.predP);
       phase_barrier();
       for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) nbr_one(i, d, c.h2, g);
       phase_barrier();
@@ -903,10 +937,10 @@ hipError_t launch_scan(Ctx& x) {
   return launch_scan_u32(x.d.cellCount, x.d.cellStart, x.ncell, x.d.blockSums, x.stream);
 }
 
-hipError_t launch_scatter_rank(Ctx& x) {
+hipError_t launch_scatter_rank(Ctx& x, const float4* pred) {
   const Dev d = dev_of(x);
   k_scatter<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d);
-  k_rank<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d);
+  k_rank<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, pred ? pred : d.predP);
   return hipGetLastError();
 }
 
@@ -957,6 +991,17 @@ hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
 
 hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel) {
   k_unpack_record<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), pos, vel);
+  return hipGetLastError();
+}
+
+hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* in) {
+  hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
+  if (e != hipSuccess) return e;
+  const Dev d = dev_of(x);
+  if (multi(x))
+    k_vort_commit_predict<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
+  else
+    k_vort_commit_predict<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
   return hipGetLastError();
 }
 
