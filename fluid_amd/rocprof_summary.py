@@ -31,11 +31,15 @@ PASS_OF = [
     ("k_scatter", "sort"),
     ("k_rank", "sort"),
     ("k_nbr_build", "neighbors"),
+    ("k_nbr_lambda", "neighbors"),      # list build fused with the first lambda
     ("k_lambda", "lambda"),
     ("k_deltap", "deltap"),
     ("k_xsph", "xsph"),
-    ("k_vort_commit", "vorticity"),
+    ("k_vort_commit", "vorticity"),     # also k_vort_commit_predict
 ]
+# Kernels that stand in for one another in a pass (one or the other runs in a
+# given step): averaged by dispatches like template variants, not added up.
+VARIANT_OF = {"k_vort_commit_predict": "k_vort_commit"}
 COPY_BYTES = 256 << 20
 
 
@@ -141,10 +145,12 @@ def counters(args) -> None:
     for p, recs in pass_recs.items():
         # Kernels of one pass that run one after the other (scan, sort) add
         # up; the template variants of one kernel (the last delta-p of a
-        # step) are averaged, weighted by their dispatches.
+        # step) and the kernels of VARIANT_OF are averaged, weighted by their
+        # dispatches.
         fam = collections.defaultdict(list)
         for k, n, rec in recs:
-            fam[re.sub(r"<.*", "", k)].append((n, rec))
+            base = re.sub(r"<.*", "", k)
+            fam[VARIANT_OF.get(base, base)].append((n, rec))
         tot = {}
         for f_ in ("hbm_read_bytes", "hbm_write_bytes", "valu_insts"):
             tot[f_] = sum(sum(n * r[f_] for n, r in v) / sum(n for n, _ in v) for v in fam.values())
