@@ -605,11 +605,17 @@ int enqueue_steps(Ctx* x, float dt, int nsteps) {
     HIPCHK(x, pbf::launch_persistent(*x, k, nsteps, x->pblock, grid));
     x->has_nbrs = true;
     x->dbg_vel = x->d.velA;
-  } else if (x->schedule == PBF_SCHED_GRAPH) {
+  } else if (x->schedule == PBF_SCHED_GRAPH && nsteps >= 3) {
+    // The first step (it predicts) and the last (it commits without
+    // predicting) are plain launches; every step between replays one
+    // captured step that starts from the prediction fused into the previous
+    // step and ends with the next one, as the flat schedule runs them.
+    int rc = run_step(x, k, 0, nullptr, false, true);
+    if (rc != PBF_OK) return rc;
     if (!x->graph_exec || x->graph_dt != dt) {
       drop_graph(x);
       HIPCHK(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = run_step(x, k, 0);
+      rc = run_step(x, k, 0, nullptr, true, true);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(x->stream, &g);
       if (rc != PBF_OK) {
@@ -621,8 +627,11 @@ int enqueue_steps(Ctx* x, float dt, int nsteps) {
       HIPCHK(x, hipGraphInstantiate(&x->graph_exec, g, nullptr, nullptr, 0));
       x->graph_dt = dt;
     }
-    for (int s = 0; s < nsteps; s++) HIPCHK(x, hipGraphLaunch(x->graph_exec, x->stream));
+    for (int s = 1; s + 1 < nsteps; s++) HIPCHK(x, hipGraphLaunch(x->graph_exec, x->stream));
+    rc = run_step(x, k, 0, nullptr, true, false);
+    if (rc != PBF_OK) return rc;
   } else {
+    // Flat schedule, and the graph schedule for calls of one or two steps.
     for (int s = 0; s < nsteps; s++) {
       const int rc = run_step(x, k, 0, nullptr, s > 0, s + 1 < nsteps);
       if (rc != PBF_OK) return rc;

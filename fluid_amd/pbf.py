@@ -89,7 +89,7 @@ class Particles:
             check(lib().pbf_step_dt(self._ctx, float(dt), nsteps), self._ctx)
 
     def set_schedule(self, schedule: str, block: int = 256) -> None:
-        """'flat', 'graph' (hipGraph-captured step) or 'persistent' (one launch,
+        """'flat', 'graph' (hipGraph-captured inner steps) or 'persistent' (one launch,
         workgroups run whole scenes); none changes the arithmetic."""
         check(lib().pbf_set_schedule(self._ctx, _lib.SCHEDULE[schedule], int(block)), self._ctx)
 

@@ -125,7 +125,8 @@ PBF_API int pbf_set_mesh(pbf_ctx *ctx, const float *pos, const float *nrm, int64
 enum {
   PBF_SCHED_FLAT = 0,       /* one launch per pass over all scenes (default) */
   PBF_SCHED_GRAPH = 1,      /* the flat launches of a step captured once in a
-                               hipGraph and replayed per step */
+                               hipGraph and replayed for every step of a call
+                               but its first and last (plain launches) */
   PBF_SCHED_PERSISTENT = 2  /* one launch for all steps: each workgroup runs
                                whole scenes, largest first, all passes */
 };

@@ -291,16 +291,18 @@ def test_schedules_bit_exact_with_flat(schedule, block):
 
 
 def test_graph_replays_each_match_eager():
-    """Replay the captured step several times with eager work (reads,
-    synchronisations) in between; every replay must equal the eager run."""
+    """Replay the captured inner step over several calls (calls of three or
+    more steps replay it, shorter calls run plain launches) with eager work
+    (reads, synchronisations) in between; every call must equal the flat
+    schedule's run."""
     p, spec, x0, v0 = _batched_items(4)
     with Particles(p, scenes=spec) as f, Particles(p, scenes=spec) as g:
         f.load(x0, v0)
         g.set_schedule("graph")
         g.load(x0, v0)
-        for _ in range(5):
-            f.time_step()
-            g.time_step()
+        for n in (4, 1, 5, 2, 3, 4):
+            f.time_step(nsteps=n)
+            g.time_step(nsteps=n)
             a, b = f.read(), g.read()
             for u, w in zip(a, b):
                 assert bits_equal(u, w)
