@@ -1,0 +1,136 @@
+"""Edge cases of the step on the GPU, each against a closed form or the
+oracle: an isolated particle (semi-implicit Euler with the reference's
+velocity update, src/particles.cpp:175-183 and :215-217), pairs at exactly
+r = h (the neighbour test is inclusive, :258-265), particle counts around
+the wave and tile sizes (1 .. 257), particles starting on the walls, and
+the neighbour-list capacity at and one chunk below the largest list."""
+import numpy as np
+import pytest
+
+from fluid_amd import scenes
+from fluid_amd.pbf import Particles, PbfError
+
+pytestmark = pytest.mark.gpu
+
+f32 = np.float32
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def run(p, x, v, nsteps):
+    with Particles(p) as c:
+        c.load(x, v)
+        c.time_step(nsteps=nsteps)
+        return c.read()
+
+
+def test_isolated_particle_is_semi_implicit_euler(orc):
+    """No neighbours: rho = 0, delta-p = 0, XSPH and confinement add
+    nothing, so each step is v.y -= g dt, x* = x + v dt, v = (x* - x) / dt,
+    x = x*, in float32 in that order."""
+    p = scenes.dam_block().params
+    x = np.array([[0.5, 0.9, 0.5]], np.float32)
+    v = np.array([[0.3, 0.0, -0.2]], np.float32)
+    steps = 20
+    gx, gv, grho = run(p, x, v, steps)
+    dt = f32(p.dt)
+    gdt = f32(p.gravity) * dt
+    inv_dt = f32(1.0) / dt
+    cx, cv = x[0].copy(), v[0].copy()
+    for _ in range(steps):
+        cv[1] = cv[1] - gdt
+        xs = cx + cv * dt
+        cv = inv_dt * (xs - cx)
+        cx = xs
+    assert bits_equal(gx[0], cx) and bits_equal(gv[0], cv)
+    assert grho[0] == 0.0
+    ox, ov, _ = orc.step(p, x, v, steps, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov)
+
+
+def test_pairs_at_exactly_h_are_neighbours(orc):
+    """h = 1/8 so that h^2 is exact; with no gravity the predicted positions
+    equal the loaded ones.  B and D sit exactly h from A (included), C one
+    float above h (excluded)."""
+    p = scenes.dam_block().params.replace(h=0.125, gravity=0.0)
+    x = np.array([
+        [0.5, 0.5, 0.5],                                      # A
+        [0.625, 0.5, 0.5],                                    # B: dx = h
+        [0.5, np.nextafter(f32(0.625), f32(1.0)), 0.5],       # C: dy = h + 1 ulp
+        [0.5, 0.5, 0.375],                                    # D: dz = -h
+    ], np.float32)
+    v = np.zeros_like(x)
+    with Particles(p) as c:
+        c.load(x, v)
+        c.step_debug("neighbors")
+        counts, ids = c.neighbors(stride=8)
+    got = [sorted(ids[i, :counts[i]].tolist()) for i in range(4)]
+    assert got == [[1, 3], [0], [], [0]]
+    og, lg = orc.neighbors(p, x, orc.NBR_GRID)
+    for i in range(4):
+        assert np.array_equal(ids[i, :counts[i]], lg[og[i]:og[i + 1]])
+    gx, gv, grho = run(p, x, v, 3)
+    ox, ov, orho = orc.step(p, x, v, 3, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
+    # poly6 and the spiky gradient vanish at r = h: A's density is 0.
+    assert grho[0] == 0.0
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 129, 257])
+def test_ragged_counts_bit_exact(orc, n):
+    """Counts around the 64-lane wave and the 256-particle workgroup."""
+    sc = scenes.dam_block()
+    rng = np.random.default_rng(n)
+    x = sc.x[rng.choice(len(sc.x), size=n, replace=False)].copy()
+    v = rng.uniform(-0.5, 0.5, size=x.shape).astype(np.float32)
+    gx, gv, grho = run(sc.params, x, v, 3)
+    ox, ov, orho = orc.step(sc.params, x, v, 3, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
+
+
+def test_particles_on_the_walls_bit_exact(orc):
+    """Particles loaded exactly on the faces, edges and corners of the box,
+    moving outwards: ray clipping, the slide and the per-axis clamp."""
+    sc = scenes.dam_block()
+    lo, hi = np.array(sc.params.box_lo, np.float32), np.array(sc.params.box_hi, np.float32)
+    rng = np.random.default_rng(7)
+    pts, vel = [], []
+    for k in range(96):
+        q = rng.uniform(lo + 0.05, hi - 0.05).astype(np.float32)
+        axes = rng.choice(3, size=1 + k % 3, replace=False)
+        w = rng.uniform(-1.0, 1.0, size=3).astype(np.float32)
+        for a in axes:
+            side = rng.integers(2)
+            q[a] = hi[a] if side else lo[a]
+            w[a] = abs(w[a]) if side else -abs(w[a])
+        pts.append(q)
+        vel.append(w)
+    x = np.array(pts, np.float32)
+    v = np.array(vel, np.float32)
+    gx, gv, grho = run(sc.params, x, v, 4)
+    ox, ov, orho = orc.step(sc.params, x, v, 4, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
+    assert np.all(gx >= lo) and np.all(gx <= hi)
+
+
+def test_list_capacity_at_and_below_the_largest_list():
+    sc = scenes.dam_block()
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.step_debug("neighbors")
+        nmax = c.neighbor_stats()[1]
+    assert nmax > 8
+    ref = run(sc.params, sc.x, sc.v, 1)
+    # Capacity = max_neighbors rounded up to whole 4-entry chunks.
+    got = run(sc.params.replace(max_neighbors=nmax), sc.x, sc.v, 1)
+    for a, b in zip(ref, got):
+        assert bits_equal(a, b)
+    with Particles(sc.params.replace(max_neighbors=4 * ((nmax - 1) // 4))) as c:
+        c.load(sc.x, sc.v)
+        with pytest.raises(PbfError) as e:
+            c.time_step()
+        assert e.value.code == -3

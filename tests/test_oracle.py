@@ -160,6 +160,28 @@ def test_isolated_particle(orc):
     assert np.all(out == xs)
 
 
+def test_isolated_particle_full_steps_float32(orc):
+    """Whole float32 steps of a lone particle: XSPH and confinement add
+    nothing, so every step is v.y -= g dt, x* = x + v dt, v = (x* - x) / dt
+    (src/particles.cpp:175-183, :215-217), x = x*, bit for bit."""
+    p = scenes.dam_block().params
+    x = np.array([[0.5, 0.9, 0.5]], np.float32)
+    v = np.array([[0.3, 0.0, -0.2]], np.float32)
+    dt = np.float32(p.dt)
+    gdt = np.float32(p.gravity) * dt
+    inv_dt = np.float32(1.0) / dt
+    cx, cv = x[0].copy(), v[0].copy()
+    for _ in range(20):
+        cv[1] = cv[1] - gdt
+        xs = cx + cv * dt
+        cv = inv_dt * (xs - cx)
+        cx = xs
+    ox, ov, orho = orc.step(p, x, v, 20, orc.NBR_GRID)
+    assert np.array_equal(ox[0].view(np.uint32), cx.view(np.uint32))
+    assert np.array_equal(ov[0].view(np.uint32), cv.view(np.uint32))
+    assert orho[0] == 0.0
+
+
 def test_lambda_denominator_is_sum_of_constraint_gradients(orc):
     """lambda_i = -C_i / (sum_k |grad_{p_k} C_i|^2 + eps) with
     grad_{p_i} C_i = (1/rho0) sum_j g_ij and grad_{p_j} C_i = -(1/rho0) g_ij
