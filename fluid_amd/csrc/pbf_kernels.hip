@@ -330,9 +330,6 @@ __device__ __forceinline__ float face_gap(float x, float lo, float cell, int k, 
 // the rounding of the cell faces), which leaves the list and its order
 // unchanged.  Candidates are tested 32 at a time, branch-free, into a hit
 // mask (self cleared afterwards); the hits are then appended in order.
-#ifndef PBF_NBR_PUSH
-#define PBF_NBR_PUSH 0  // 1: one 4-byte store per hit; 0: 16-byte chunk stores
-#endif
 __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, const SceneGrid& g) {
   const float4 xi = d.predQ[i];
   int cc[3];
@@ -342,14 +339,6 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
   const float4* __restrict__ pred = d.predQ;
   const uint32_t cap = (uint32_t)d.K4 * kChunk;
   uint32_t cnt = 0;
-#if PBF_NBR_PUSH
-  // Slot k of particle i: chunk k/4 (chunks 64*4 entries apart), entry k%4.
-  uint32_t* const base = d.nbr + nbr_chunk(i, 0, d.K4);
-  auto push = [&](uint32_t j) {
-    if (cnt < cap) base[(size_t)(cnt >> 2) * (64u * kChunk) + (cnt & 3u)] = j;
-    cnt++;
-  };
-#else
   // A 4-entry shift register: the latest hit enters at b3, and after every
   // fourth hit b0..b3 hold the chunk in order.
   uint32_t b0 = 0u, b1 = 0u, b2 = 0u, b3 = 0u;
@@ -365,7 +354,6 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
         *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) = make_uint4(b0, b1, b2, b3);
     }
   };
-#endif
   auto hit = [&](const float4& xj) -> uint32_t {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float d2 = rx * rx + ry * ry + rz * rz;
@@ -411,7 +399,6 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
       }
     }
   }
-#if !PBF_NBR_PUSH
   // Partial last chunk: its r = cnt % 4 entries are b(4-r)..b3.  The slots
   // past them are filled with a repeat of the first, so every slot of the
   // chunks in use holds a particle index.
@@ -424,7 +411,6 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
     if (chunk < d.K4)
       *reinterpret_cast<uint4*>(d.nbr + nbr_chunk(i, chunk, d.K4)) = make_uint4(e0, e1, e2, e0);
   }
-#endif
   if (cnt > cap) {
     atomicMax(&d.flags[0], cnt);
     cnt = cap;
