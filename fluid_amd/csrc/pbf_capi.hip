@@ -176,8 +176,9 @@ int verify_math(Ctx* x) {
     it = v.div.emplace(key, e_lo < e_hi && hb[0] == 0).first;
   }
   // The arguments the kernels pass must lie in the checked ranges: spiky's
-  // reciprocal argument is h^6 |r| with EPS_D <= |r| < h (or 1), and square
-  // roots below 2^-47 are only compared with EPS_D.
+  // reciprocal argument is h^6 |r| with EPS_D <= |r| < h (or 1), and the
+  // short square root of an argument below 2^-96 (NaN or under 2^-47) only
+  // meets the EPS_D cut-off.
   const float zlo = k.h6 * k.eps_d, zhi = k.h6 * k.h;
   x->fast_div = it->second && div_range;
   x->fast_rcp = v.rcp && zlo >= 0x1p-100f && zhi < 0x1p100f;

@@ -389,29 +389,26 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // range; otherwise the IEEE expansion runs.  The flags are kernel arguments,
 // so the choice is a uniform branch.
 //
-// sqrt_rn: the hardware square root (within 1 ulp) corrected by the sign of
-//   the residuals x - s'*s for the neighbours s' = s -+ 1 ulp.  Checked on 0
-//   and [2^-96, 2^128).  Below 2^-96 (no pre-scaling) the root may be off by
-//   an ulp; those roots are below 2^-47 and the callers only compare them
-//   with EPS_D, which the host requires to be at least 2^-47.
+// sqrt_rn: from the hardware reciprocal square root y ~ 1/sqrt(x), the root
+//   s = x*y corrected once by its residual: s + (x - s*s) * y/2 (the Newton
+//   step for s^2 = x, with FMA).  Checked on [2^-96, 2^128).  At 0 it gives
+//   NaN (0 * inf) and below 2^-96 NaN or a value below 2^-47 (also
+//   checked); spiky_sel treats both as under EPS_D, which the host requires
+//   to be at least 2^-47, as the IEEE root is.
 __device__ __forceinline__ float sqrt_rn(float x) {
-  const float s = __builtin_amdgcn_sqrtf(x);
-  const float sdn = __int_as_float(__float_as_int(s) - 1);
-  const float sup = __int_as_float(__float_as_int(s) + 1);
-  const float rdn = fmaf(-sdn, s, x);
-  const float rup = fmaf(-sup, s, x);
-  const float t = (rdn <= 0.0f) ? sdn : s;
-  return (rup > 0.0f) ? sup : t;
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float s = x * y;
+  const float hy = 0.5f * y;
+  const float r = fmaf(-s, s, x);
+  return fmaf(r, hy, s);
 }
 
-// rcp_seq: hardware reciprocal refined by two Newton steps with FMA.
+// rcp_seq: hardware reciprocal refined by one Newton step with FMA.
 //   Checked on [2^-100, 2^100]; the spiky kernel passes h^6 |r| with
 //   EPS_D <= |r| < h, or 1.
 __device__ __forceinline__ float rcp_seq(float z) {
-  float y = __builtin_amdgcn_rcpf(z);
-  float e = fmaf(-z, y, 1.0f);
-  y = fmaf(e, y, y);
-  e = fmaf(-z, y, 1.0f);
+  const float y = __builtin_amdgcn_rcpf(z);
+  const float e = fmaf(-z, y, 1.0f);
   return fmaf(e, y, y);
 }
 
@@ -444,7 +441,8 @@ __device__ __forceinline__ float poly6_sel(const SimConst& c, float r2) {
 __device__ __forceinline__ float3 spiky_sel(const SimConst& c, float rx, float ry, float rz,
                                             float r2) {
   const float rl = c.fast_sqrt ? sqrt_rn(r2) : sqrtf(r2);
-  const bool zero = (rl >= c.h) || (rl < c.eps_d);
+  // (rl >= h) || (rl < EPS_D), written to hold for a NaN root as well.
+  const bool zero = !(rl < c.h) || !(rl >= c.eps_d);
   const float q = c.h - rl;
   const float coef = zero ? 0.0f : c.m3c2 * (q * q);
   const float z = zero ? 1.0f : c.h6 * rl;
