@@ -109,8 +109,21 @@ def calibration(table, counter: str) -> float:
     return COPY_BYTES / best
 
 
+def gpu_shape(path: str):
+    """(CUs, XCDs) of the profiled GPU from rocprofv3's agent table."""
+    for r in read_csv(os.path.join(path, "run_agent_info.csv")):
+        if r.get("Name", "").startswith("gfx"):
+            return int(r["Cu_Count"]), int(r["Num_Xcc"])
+    raise SystemExit(f"no GPU agent in {path}/run_agent_info.csv")
+
+
 def counters(args) -> None:
     fetch, write, sq = per_kernel(args.fetch), per_kernel(args.write), per_kernel(args.sq)
+    # Optional pass: TA_TA_BUSY_sum (address unit busy cycles, summed over
+    # the CUs) against GRBM_GUI_ACTIVE (GPU busy cycles, summed over the
+    # XCDs): the fraction of the launch each CU's address unit was busy.
+    ta = per_kernel(args.ta) if args.ta else {}
+    ncu, nxcd = gpu_shape(args.ta) if args.ta else (0, 0)
     dur = durations(args.sq)
     ff, wf = calibration(fetch, "FETCH_SIZE"), calibration(write, "WRITE_SIZE")
     out = {"workload": args.workload, "fetch_factor": ff, "write_factor": wf, "passes": {},
@@ -121,7 +134,7 @@ def counters(args) -> None:
              f"the 256 MiB calibration copy counts {COPY_BYTES / ff:.0f} KiB FETCH_SIZE and "
              f"{COPY_BYTES / wf:.0f} KiB WRITE_SIZE",
              "", f"{'kernel':28s} {'grid':>8s} {'us':>7s} {'HBM rd MB':>10s} {'HBM wr MB':>10s} "
-             f"{'VALU Minst':>10s} {'waves':>7s} {'VMEM rd M':>9s} {'LDS bank cf':>11s}"]
+             f"{'VALU Minst':>10s} {'waves':>7s} {'VMEM rd M':>9s} {'LDS bank cf':>11s} {'TA busy':>7s}"]
     pass_recs = collections.defaultdict(list)
     for key in sorted(sq, key=lambda k: -sq[k].get("SQ_INSTS_VALU", 0)):
         k, grid = key
@@ -136,11 +149,15 @@ def counters(args) -> None:
                "waves": s.get("SQ_WAVES", 0.0), "vmem_rd_insts": s.get("SQ_INSTS_VMEM_RD", 0.0),
                "lds_bank_conflict": s.get("SQ_LDS_BANK_CONFLICT", 0.0),
                "lds_idx_active": s.get("SQ_LDS_IDX_ACTIVE", 0.0)}
+        t = ta.get(key, {})
+        if t.get("GRBM_GUI_ACTIVE"):
+            rec["ta_busy_frac"] = t["TA_TA_BUSY_sum"] / ncu / (t["GRBM_GUI_ACTIVE"] / nxcd)
         out["kernels"][f"{k}[{grid}]"] = rec
         us = f"{rec['us']:7.1f}" if rec["us"] else "      -"
+        tb = f"{rec['ta_busy_frac']:7.2f}" if "ta_busy_frac" in rec else "      -"
         lines.append(f"{k[:28]:28s} {grid:>8s} {us} {rd / 1e6:10.1f} {wr / 1e6:10.1f} "
                      f"{rec['valu_insts'] / 1e6:10.1f} {rec['waves']:7.0f} "
-                     f"{rec['vmem_rd_insts'] / 1e6:9.2f} {rec['lds_bank_conflict']:11.0f}")
+                     f"{rec['vmem_rd_insts'] / 1e6:9.2f} {rec['lds_bank_conflict']:11.0f} {tb}")
         pass_recs[p].append((k, s["dispatches"], rec))
     for p, recs in pass_recs.items():
         # Kernels of one pass that run one after the other (scan, sort) add
@@ -155,10 +172,14 @@ def counters(args) -> None:
         for f_ in ("hbm_read_bytes", "hbm_write_bytes", "valu_insts"):
             tot[f_] = sum(sum(n * r[f_] for n, r in v) / sum(n for n, _ in v) for v in fam.values())
         tot["hbm_bytes"] = tot["hbm_read_bytes"] + tot["hbm_write_bytes"]
+        tas = [(n, r["ta_busy_frac"]) for _, n, r in recs if "ta_busy_frac" in r]
+        if tas:
+            tot["ta_busy_frac"] = sum(n * f_ for n, f_ in tas) / sum(n for n, _ in tas)
         out["passes"][p] = tot
     lines += ["", "per launch of each pass (HBM bytes = calibrated FETCH + WRITE):"]
     for p, t in out["passes"].items():
-        lines.append(f"  {p:10s} HBM {t['hbm_bytes'] / 1e6:8.1f} MB   VALU {t['valu_insts'] / 1e6:7.1f} M wave-instructions")
+        ta_s = f"   TA busy {t['ta_busy_frac']:.2f}" if "ta_busy_frac" in t else ""
+        lines.append(f"  {p:10s} HBM {t['hbm_bytes'] / 1e6:8.1f} MB   VALU {t['valu_insts'] / 1e6:7.1f} M wave-instructions{ta_s}")
     with open(args.out_json, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     with open(args.out_txt, "w") as f:
@@ -178,6 +199,7 @@ def main() -> None:
     b.add_argument("--fetch", required=True)
     b.add_argument("--write", required=True)
     b.add_argument("--sq", required=True)
+    b.add_argument("--ta", default=None, help="run dir with TA_TA_BUSY_sum and GRBM_GUI_ACTIVE")
     b.add_argument("--workload", required=True)
     b.add_argument("--command", required=True)
     b.add_argument("--out-json", required=True)
