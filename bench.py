@@ -15,11 +15,9 @@ HBM.  Prints ONE JSON line:
              8 TB/s.  `traffic` is the HBM bytes of one launch from the
              committed rocprofv3 counters (calibrated FETCH_SIZE +
              WRITE_SIZE); `valu` is the issue roof of these passes:
-             wave-level VALU instructions of one launch
-             (committed counters) x 64 lanes / mean launch time, against
-             256 CU x 64 lanes x 2.4 GHz; `address_unit_busy_frac` the
-             co-bound: the fraction of the launch each CU's vector-memory
-             address unit was busy (committed counters);
+             wave-level VALU instructions of one launch (committed
+             counters) x 64 lanes / mean launch time, against 256 CU x 64
+             lanes x 2.4 GHz;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
@@ -326,10 +324,6 @@ def main() -> int:
             "frac": lane_ops / VALU_PEAK_LANE_OPS,
             "valu_insts_per_launch": cp["valu_insts"],
         }
-    if cp.get("ta_busy_frac") is not None:
-        # The co-bound: each CU's vector-memory address unit (TA) busy over
-        # the launch (rocprofv3 TA_TA_BUSY_sum / CUs / GRBM_GUI_ACTIVE per XCD).
-        roof["address_unit_busy_frac"] = cp["ta_busy_frac"]
     for name, pd in passes.items():
         c = counters.get(name, {})
         if c.get("hbm_bytes"):
@@ -337,6 +331,8 @@ def main() -> int:
         if c.get("valu_insts"):
             pd["valu_frac"] = c["valu_insts"] * 64 / (pd["us_per_launch"] * 1e-6) / VALU_PEAK_LANE_OPS
         if c.get("ta_busy_frac") is not None:
+            # Share of the launch with gathers in flight in each CU's
+            # address unit (TA_TA_BUSY_sum / CUs / GRBM_GUI_ACTIVE per XCD).
             pd["ta_busy_frac"] = c["ta_busy_frac"]
     out = {
         "metric": "PBF particle-steps/s @4M",
