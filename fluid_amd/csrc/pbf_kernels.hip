@@ -5,11 +5,15 @@
 //   scan           wave64 exclusive scan of the cell histogram
 //   scatter/rank   counting sort by cell, deterministic in-cell rank by id,
 //                  gather into sorted order
-//   nbr_build      27-cell search -> chunked ELL neighbour lists (fixed per step)
-//   lambda         density / lambda pass            (x S)
-//   deltap         delta-p + clamp pass              (x S; last one also v)
+//   nbr_lambda     27-cell search -> chunked ELL neighbour lists (fixed per
+//                  step), then the first density / lambda pass in the same
+//                  thread
+//   lambda         density / lambda pass            (S - 1 more)
+//   deltap         delta-p + clamp pass              (x S; the last one
+//                  writes the (x*, v) records)
 //   xsph           XSPH viscosity, vorticity, density
-//   vort_commit    vorticity confinement, x <- x*
+//   vort_commit    vorticity confinement, x <- x*; between the steps of one
+//                  call fused with the next step's predict
 // Particles stay in the order (cell, id), so the lists of a particle are in
 // ascending sorted index, i.e. the same order a CPU visits the 27 cells in.
 //
@@ -35,7 +39,7 @@ constexpr int kBlock = 256;
 #define PBF_UNROLL_XSPH 8
 #endif
 constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
-constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers two float4 per neighbour
+constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers a 24-byte record per neighbour
 
 // Device pointers and sizes of one context, passed by value to every kernel.
 struct Dev {
