@@ -98,10 +98,21 @@ struct AdjDev {
   uint32_t n;
 };
 
+// Visits the neighbours of i in list order.  A whole 4-entry chunk is read
+// with one load and its four bodies are laid out one after the other, so the
+// gathers of the four (they only read) can be issued together.
 template <class Body>
 __device__ __forceinline__ void nbrs(const AdjDev& a, uint32_t i, Body body) {
   const int cnt = (int)a.ncnt[i];
-  for (int k = 0; k < cnt; k++) body(a.nbr[nbr_slot(i, k, a.K4)]);
+  const int nfull = cnt >> 2;
+  for (int ch = 0; ch < nfull; ch++) {
+    const uint4 q = *reinterpret_cast<const uint4*>(a.nbr + nbr_chunk(i, ch, a.K4));
+    body(q.x);
+    body(q.y);
+    body(q.z);
+    body(q.w);
+  }
+  for (int k = nfull * 4; k < cnt; k++) body(a.nbr[nbr_slot(i, k, a.K4)]);
 }
 
 __device__ __forceinline__ uint32_t idx() { return blockIdx.x * kB + threadIdx.x; }
@@ -332,6 +343,26 @@ __global__ __launch_bounds__(kB) void k_adj_predict(AdjDev a, SimConst c,
   vbar_id[id] = make_float4(vb.x, vb.y, vb.z, 0.f);
 }
 
+// Deterministic sum of n doubles in two launches with a fixed order:
+// k_sum_double_part gives workgroup b the sum of slice b of kSumParts equal
+// slices, then k_sum_double adds the kSumParts partial sums.
+constexpr int kSumParts = 64;
+__global__ __launch_bounds__(kB) void k_sum_double_part(uint32_t n, const double* __restrict__ in,
+                                                        double* __restrict__ part) {
+  __shared__ double sh[kB];
+  const uint32_t per = (n + kSumParts - 1) / kSumParts;
+  const uint32_t b0 = blockIdx.x * per, b1 = min(n, b0 + per);
+  double s = 0.0;
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += kB) s += in[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // Deterministic sum of n doubles (one workgroup, fixed order).
 __global__ __launch_bounds__(kB) void k_sum_double(uint32_t n, const double* __restrict__ in,
                                                    double* __restrict__ out) {
@@ -366,6 +397,7 @@ struct pbf_adj {
   float4 *yb = nullptr, *yb2 = nullptr, *xst = nullptr, *xsacc = nullptr, *vb3 = nullptr;
   float4 *etab = nullptr, *wb = nullptr, *aux = nullptr, *Gb = nullptr, *xbn = nullptr;
   double *kpart = nullptr, *ksteps = nullptr;
+  double* kslice = nullptr;  // partial sums of the dL/dk reduction
   uint32_t* contacts = nullptr;
   double backward_ms = 0.0;
   int64_t last_contacts = 0;
@@ -402,6 +434,7 @@ void free_all(pbf_adj* a) {
   afree(a->xbar); afree(a->vbar); afree(a->yb); afree(a->yb2); afree(a->xst);
   afree(a->xsacc); afree(a->vb3); afree(a->etab); afree(a->wb); afree(a->aux);
   afree(a->Gb); afree(a->xbn); afree(a->kpart); afree(a->ksteps); afree(a->contacts);
+  afree(a->kslice);
 }
 
 // Loads a load-order state (x, v) into the context as its committed state.
@@ -475,7 +508,6 @@ int backward_step(pbf_adj* a, const pbf::SimConst& k, int t) {
                                          a->xst, a->contacts, g);
   AHIP(a, hipGetLastError());
   const int S = c->p.iterations;
-  AHIP(a, pbf::launch_fill_u32(reinterpret_cast<uint32_t*>(a->kpart), 2 * np * (size_t)S, 0u, s));
   for (int it = S - 1; it >= 0; it--) {
     const float4* xs = a->stash + (size_t)it * np;
     pbf::k_adj_lambda<<<nb, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->contacts, g);
@@ -485,7 +517,9 @@ int backward_step(pbf_adj* a, const pbf::SimConst& k, int t) {
     std::swap(a->yb, a->yb2);
   }
   // dL/dk of this step: one fixed-order sum over every iteration's partials.
-  pbf::k_sum_double<<<1, pbf::kB, 0, s>>>((uint32_t)(np * (size_t)S), a->kpart, a->ksteps + t);
+  pbf::k_sum_double_part<<<pbf::kSumParts, pbf::kB, 0, s>>>((uint32_t)(np * (size_t)S), a->kpart,
+                                                            a->kslice);
+  pbf::k_sum_double<<<1, pbf::kB, 0, s>>>((uint32_t)pbf::kSumParts, a->kslice, a->ksteps + t);
   pbf::k_adj_predict<<<nb, pbf::kB, 0, s>>>(ad, k, a->yb, a->xst, a->xbar, a->vbar);
   AHIP(a, hipGetLastError());
   return PBF_OK;
@@ -525,6 +559,10 @@ PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32
   AA(a->xsacc, np); AA(a->vb3, np); AA(a->etab, np); AA(a->wb, np); AA(a->aux, np);
   AA(a->Gb, np); AA(a->xbn, np);
   AA(a->kpart, np * (size_t)S); AA(a->ksteps, 2 * (size_t)nsteps); AA(a->contacts, 4);
+  AA(a->kslice, (size_t)pbf::kSumParts);
+  // The partials of the padding slots n..npad-1 are never written: zero
+  // them once.
+  if (e == hipSuccess) e = hipMemset(a->kpart, 0, np * (size_t)S * sizeof(double));
 #undef AA
   if (e != hipSuccess) {
     free_all(a);
