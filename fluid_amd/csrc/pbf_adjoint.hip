@@ -384,6 +384,16 @@ unsigned nblk(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
 }  // namespace pbf
 
 // ------------------------------------------------------------- host side --
+// What the backward pass of one step reads of its forward pass, all in the
+// step's sorted order: the sorted ids and state (posB, velB after the
+// solve), x* after the solve (predQ) and after XSPH with |w| (predP), w
+// (vort), the lists, and x*_m for m = 0..S (stash).
+struct AdjTape {
+  float4 *posB = nullptr, *predP = nullptr, *predQ = nullptr, *vort = nullptr, *velB = nullptr;
+  float4* stash = nullptr;
+  uint32_t *nbr = nullptr, *ncnt = nullptr;
+};
+
 struct pbf_adj {
   pbf_ctx* ctx = nullptr;  // plain context that runs the forward passes
   int nsteps = 0, every = 1, nckpt = 0;
@@ -391,8 +401,7 @@ struct pbf_adj {
   float k = 0.0f;
   bool have_forward = false;
   float4 *ckx = nullptr, *ckv = nullptr;      // checkpoints, load order
-  float4 *segx = nullptr, *segv = nullptr;    // states of one segment, load order
-  float4* stash = nullptr;                    // x*_m of one step, m = 0..S (sorted)
+  std::vector<AdjTape> tape;                  // one slot per step of a segment
   float4 *xbar = nullptr, *vbar = nullptr;    // running adjoints, load order
   float4 *yb = nullptr, *yb2 = nullptr, *xst = nullptr, *xsacc = nullptr, *vb3 = nullptr;
   float4 *etab = nullptr, *wb = nullptr, *aux = nullptr, *Gb = nullptr, *xbn = nullptr;
@@ -430,7 +439,11 @@ void afree(T*& p) {
 }
 
 void free_all(pbf_adj* a) {
-  afree(a->ckx); afree(a->ckv); afree(a->segx); afree(a->segv); afree(a->stash);
+  afree(a->ckx); afree(a->ckv);
+  for (AdjTape& t : a->tape) {
+    afree(t.posB); afree(t.predP); afree(t.predQ); afree(t.vort); afree(t.velB);
+    afree(t.stash); afree(t.nbr); afree(t.ncnt);
+  }
   afree(a->xbar); afree(a->vbar); afree(a->yb); afree(a->yb2); afree(a->xst);
   afree(a->xsacc); afree(a->vb3); afree(a->etab); afree(a->wb); afree(a->aux);
   afree(a->Gb); afree(a->xbn); afree(a->kpart); afree(a->ksteps); afree(a->contacts);
@@ -455,45 +468,60 @@ int save_state(pbf_adj* a, float4* x, float4* v) {
   return PBF_OK;
 }
 
-// One forward step; with stash, also records x*_m for m = 0..S (sorted) and
-// stops after the XSPH pass (the vorticity pass is recomputed by the adjoint).
-int forward_step(pbf_adj* a, const pbf::SimConst& k, bool stash) {
+// One forward step from the context's committed state.  With a tape slot,
+// the step's sorted state, lists and x*_m (m = 0..S) are written straight
+// into the slot (the context's buffers are swapped for the slot's during
+// the step) for the backward pass to read.
+int forward_step(pbf_adj* a, const pbf::SimConst& k, AdjTape* slot) {
   pbf::Ctx* c = X(a);
   pbf::DevState& d = c->d;
   const uint32_t n = (uint32_t)a->n;
   const size_t np = (size_t)a->npad;
-  AHIP(a, pbf::launch_predict(*c, k));
-  AHIP(a, pbf::launch_scan(*c));
-  AHIP(a, pbf::launch_scatter_rank(*c));
-  AHIP(a, pbf::launch_nbr_build(*c, k));
-  if (stash) pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, a->stash);
-  for (int it = 0; it < c->p.iterations; it++) {
-    const bool last = it == c->p.iterations - 1;
-    AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP));
-    AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, last));
-    // The last iteration writes x* and v as XSPH records; the adjoint reads
-    // them from predQ and velB.
-    if (last) AHIP(a, pbf::launch_unpack_record(*c, d.predQ, d.velB));
-    if (stash)
-      pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ,
-                                                           a->stash + (size_t)(it + 1) * np);
+  const pbf::DevState own = d;
+  if (slot) {
+    d.posB = slot->posB;
+    d.predP = slot->predP;
+    d.predQ = slot->predQ;
+    d.vort = slot->vort;
+    d.velB = slot->velB;
+    d.nbr = slot->nbr;
+    d.ncnt = slot->ncnt;
   }
-  AHIP(a, pbf::launch_xsph(*c, k, d.predP));
-  if (!stash) AHIP(a, pbf::launch_vort_commit(*c, k, d.predP));
-  AHIP(a, hipGetLastError());
-  return PBF_OK;
+  auto run = [&]() -> int {
+    AHIP(a, pbf::launch_predict(*c, k));
+    AHIP(a, pbf::launch_scan(*c));
+    AHIP(a, pbf::launch_scatter_rank(*c));
+    AHIP(a, pbf::launch_nbr_build(*c, k));
+    if (slot) pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, slot->stash);
+    for (int it = 0; it < c->p.iterations; it++) {
+      const bool last = it == c->p.iterations - 1;
+      AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP));
+      AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, last));
+      // The last iteration writes x* and v as XSPH records; the adjoint reads
+      // them from predQ and velB.
+      if (last) AHIP(a, pbf::launch_unpack_record(*c, d.predQ, d.velB));
+      if (slot)
+        pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ,
+                                                             slot->stash + (size_t)(it + 1) * np);
+    }
+    AHIP(a, pbf::launch_xsph(*c, k, d.predP));
+    AHIP(a, pbf::launch_vort_commit(*c, k, d.predP));
+    AHIP(a, hipGetLastError());
+    return PBF_OK;
+  };
+  const int rc = run();
+  d = own;
+  return rc;
 }
 
-// Backward of one step whose starting state is loaded in the context.
+// Backward of step t, whose forward pass is recorded in tape slot T.
 // xbar/vbar (load order) hold dL/d(state after the step) on entry and
 // dL/d(state before the step) on exit; ksteps[t] receives this step's dL/dk.
-int backward_step(pbf_adj* a, const pbf::SimConst& k, int t) {
+int backward_step(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
   pbf::Ctx* c = X(a);
-  pbf::DevState& d = c->d;
   const uint32_t n = (uint32_t)a->n;
   const size_t np = (size_t)a->npad;
-  int rc = forward_step(a, k, true);
-  if (rc != PBF_OK) return rc;
+  const AdjTape d = T;
   pbf::AdjDev ad{d.posB, d.nbr, d.ncnt, pbf::k4_of(*c), n};
   const unsigned nb = pbf::nblk(n);
   hipStream_t s = c->stream;
@@ -509,7 +537,7 @@ int backward_step(pbf_adj* a, const pbf::SimConst& k, int t) {
   AHIP(a, hipGetLastError());
   const int S = c->p.iterations;
   for (int it = S - 1; it >= 0; it--) {
-    const float4* xs = a->stash + (size_t)it * np;
+    const float4* xs = d.stash + (size_t)it * np;
     pbf::k_adj_lambda<<<nb, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->contacts, g);
     pbf::k_adj_pos<<<nb, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->yb2,
                                           a->kpart + (size_t)it * np);
@@ -553,8 +581,13 @@ PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32
 #define AA(ptr, cnt) \
   if (e == hipSuccess) e = hipMalloc((void**)&ptr, (cnt) * sizeof(*ptr))
   AA(a->ckx, np * a->nckpt); AA(a->ckv, np * a->nckpt);
-  AA(a->segx, np * every); AA(a->segv, np * every);
-  AA(a->stash, np * (S + 1));
+  // The tape: one slot per step of a checkpoint segment.
+  const size_t K = (size_t)pbf::k4_of(*X(a)) * pbf::kChunk;
+  a->tape.resize((size_t)every);
+  for (AdjTape& t : a->tape) {
+    AA(t.posB, np); AA(t.predP, np + 64); AA(t.predQ, np + 64); AA(t.vort, np); AA(t.velB, np);
+    AA(t.stash, np * (S + 1)); AA(t.nbr, np * K); AA(t.ncnt, np);
+  }
   AA(a->xbar, np); AA(a->vbar, np); AA(a->yb, np); AA(a->yb2, np); AA(a->xst, np);
   AA(a->xsacc, np); AA(a->vb3, np); AA(a->etab, np); AA(a->wb, np); AA(a->aux, np);
   AA(a->Gb, np); AA(a->xbn, np);
@@ -603,7 +636,7 @@ PBF_API int pbf_adj_forward(pbf_adj* a, const float* x0, const float* v0, float 
       const size_t o = (size_t)(t / a->every) * (size_t)a->npad;
       if ((rc = save_state(a, a->ckx + o, a->ckv + o)) != PBF_OK) return rc;
     }
-    if ((rc = forward_step(a, k, false)) != PBF_OK) return rc;
+    if ((rc = forward_step(a, k, nullptr)) != PBF_OK) return rc;
   }
   rc = pbf_sync(a->ctx);
   if (rc != PBF_OK) return afail(a, rc, pbf_last_error(a->ctx));
@@ -640,20 +673,14 @@ PBF_API int pbf_adj_backward(pbf_adj* a, const float* xbar_T, const float* vbar_
   for (int seg = a->nckpt - 1; seg >= 0; seg--) {
     const int t0 = seg * a->every;
     const int len = std::min(a->every, a->nsteps - t0);
-    // Recompute the states of the segment from its checkpoint.
+    // Replay the segment from its checkpoint, recording every step.
     if ((rc = load_state(a, a->ckx + (size_t)seg * np, a->ckv + (size_t)seg * np)) != PBF_OK)
       return rc;
-    for (int t = 0; t < len; t++) {
-      if ((rc = save_state(a, a->segx + (size_t)t * np, a->segv + (size_t)t * np)) != PBF_OK)
-        return rc;
-      if (t + 1 < len && (rc = forward_step(a, k, false)) != PBF_OK) return rc;
-    }
+    for (int t = 0; t < len; t++)
+      if ((rc = forward_step(a, k, &a->tape[(size_t)t])) != PBF_OK) return rc;
     // Reverse through the segment.
-    for (int t = len - 1; t >= 0; t--) {
-      if ((rc = load_state(a, a->segx + (size_t)t * np, a->segv + (size_t)t * np)) != PBF_OK)
-        return rc;
-      if ((rc = backward_step(a, k, t0 + t)) != PBF_OK) return rc;
-    }
+    for (int t = len - 1; t >= 0; t--)
+      if ((rc = backward_step(a, k, t0 + t, a->tape[(size_t)t])) != PBF_OK) return rc;
   }
   AHIP(a, hipEventRecord(e1, s));
   AHIP(a, hipStreamSynchronize(s));

@@ -198,10 +198,12 @@ PBF_API const char *pbf_last_error(const pbf_ctx *ctx);
 /* ---- differentiable extension (no counterpart in the reference) --------
  * Reverse-mode derivatives of nsteps steps of one scene (collision treated
  * as the identity; pbf_adj_stats counts wall contacts).  The forward pass
- * keeps (x, v) every checkpoint_every steps; the backward pass recomputes
- * each segment from its checkpoint, rebuilding the neighbour lists, and runs
- * hand-written adjoint kernels of the lambda, delta-p, XSPH and vorticity
- * passes. */
+ * keeps (x, v) every checkpoint_every steps; the backward pass replays each
+ * segment once from its checkpoint, recording every step's sorted state,
+ * neighbour lists and solver iterates in a device tape of checkpoint_every
+ * slots (about (5 + 4 (iterations + 1)) * 16 + 4 (max_neighbors + 1) bytes
+ * per particle each), then runs hand-written adjoint kernels of the lambda,
+ * delta-p, XSPH and vorticity passes over the tape in reverse. */
 typedef struct pbf_adj pbf_adj;
 PBF_API int pbf_adj_create(const pbf_params *p, int64_t n, int32_t nsteps,
                            int32_t checkpoint_every, pbf_adj **out);
