@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -98,24 +99,40 @@ struct AdjDev {
   uint32_t n;
 };
 
-// Visits the neighbours of i in list order.  A whole 4-entry chunk is read
-// with one load and its four bodies are laid out one after the other, so the
-// gathers of the four (they only read) can be issued together.
-template <class Body>
-__device__ __forceinline__ void nbrs(const AdjDev& a, uint32_t i, Body body) {
+__device__ __forceinline__ uint32_t idx() { return blockIdx.x * kB + threadIdx.x; }
+
+// The adjoint's neighbour passes can give each particle L consecutive lanes
+// (L = 1, 2, 4): lane q of particle i visits the chunks ch = q, q + L, ...
+// of its list (and the partial last chunk if it falls to q) and the L
+// partial sums are added with a fixed butterfly, so the result is the same
+// on every run.  Small scenes use L = 4 to put four times as many waves on
+// the chip; a lane's sums then hold a quarter of the terms.
+template <int L, class Body>
+__device__ __forceinline__ void nbrs_split(const AdjDev& a, uint32_t i, int q, Body body) {
   const int cnt = (int)a.ncnt[i];
   const int nfull = cnt >> 2;
-  for (int ch = 0; ch < nfull; ch++) {
-    const uint4 q = *reinterpret_cast<const uint4*>(a.nbr + nbr_chunk(i, ch, a.K4));
-    body(q.x);
-    body(q.y);
-    body(q.z);
-    body(q.w);
+  for (int ch = q; ch < nfull; ch += L) {
+    const uint4 e = *reinterpret_cast<const uint4*>(a.nbr + nbr_chunk(i, ch, a.K4));
+    body(e.x);
+    body(e.y);
+    body(e.z);
+    body(e.w);
   }
-  for (int k = nfull * 4; k < cnt; k++) body(a.nbr[nbr_slot(i, k, a.K4)]);
+  if (q == nfull % L)
+    for (int k = nfull * 4; k < cnt; k++) body(a.nbr[nbr_slot(i, k, a.K4)]);
 }
 
-__device__ __forceinline__ uint32_t idx() { return blockIdx.x * kB + threadIdx.x; }
+// Sum over the L lanes of a particle (every lane gets the same value:
+// floating-point addition is commutative).
+template <int L>
+__device__ __forceinline__ float lane_sum(float v) {
+  for (int off = L / 2; off >= 1; off >>= 1) v = v + __shfl_xor(v, off);
+  return v;
+}
+template <int L>
+__device__ __forceinline__ float3 lane_sum(float3 v) {
+  return make_float3(lane_sum<L>(v.x), lane_sum<L>(v.y), lane_sum<L>(v.z));
+}
 
 __global__ __launch_bounds__(kB) void k_load_state(uint32_t n, const float4* __restrict__ x,
                                                    const float4* __restrict__ v,
@@ -150,21 +167,25 @@ __global__ __launch_bounds__(kB) void k_copy4(uint32_t n, const float4* __restri
 
 // Vorticity confinement, part 1: eta_bar and the direct part of w_bar.
 // xw = (x*_S, |w|), w = (w, .), vb4 = dL/dv_new.
+template <int L>
 __global__ __launch_bounds__(kB) void k_adj_vort_a(AdjDev a, SimConst c,
                                                    const float4* __restrict__ xw,
                                                    const float4* __restrict__ w,
                                                    const float4* __restrict__ vb4,
                                                    float4* __restrict__ etabar,
                                                    float4* __restrict__ wbar) {
-  const uint32_t i = idx();
+  const uint32_t i = idx() / L;
+  const int lq = (int)(idx() % L);
   if (i >= a.n) return;
   const float3 xi = f3(xw[i]);
   float3 eta = make_float3(0.f, 0.f, 0.f);
-  nbrs(a, i, [&](uint32_t j) {
+  nbrs_split<L>(a, i, lq, [&](uint32_t j) {
     const float4 q = xw[j];
     const Pair p = pair_of(c, xi, f3(q));
     eta = eta + q.w * p.g;
   });
+  eta = lane_sum<L>(eta);
+  if (lq != 0) return;
   const float e2 = dot3(eta, eta);
   float3 eb = make_float3(0.f, 0.f, 0.f), wb = make_float3(0.f, 0.f, 0.f);
   if (sqrtf(e2) > c.eps_d) {
@@ -183,6 +204,7 @@ __global__ __launch_bounds__(kB) void k_adj_vort_a(AdjDev a, SimConst c,
 // Vorticity confinement, part 2: w_bar through |w_j| in eta_j, and x*_bar
 // through g in eta: H_ij (|w_j| eta_bar_i - |w_i| eta_bar_j).  xs_acc starts
 // from dL/dx_new (x_new = x*_S).
+template <int L>
 __global__ __launch_bounds__(kB) void k_adj_vort_b(AdjDev a, SimConst c,
                                                    const float4* __restrict__ xw,
                                                    const float4* __restrict__ w,
@@ -190,7 +212,8 @@ __global__ __launch_bounds__(kB) void k_adj_vort_b(AdjDev a, SimConst c,
                                                    float4* __restrict__ wbar,
                                                    const float4* __restrict__ xb_new,
                                                    float4* __restrict__ xs_acc) {
-  const uint32_t i = idx();
+  const uint32_t i = idx() / L;
+  const int lq = (int)(idx() % L);
   if (i >= a.n) return;
   const float4 xwi = xw[i];
   const float3 xi = f3(xwi);
@@ -198,13 +221,16 @@ __global__ __launch_bounds__(kB) void k_adj_vort_b(AdjDev a, SimConst c,
   const float3 ebi = f3(etabar[i]);
   float s = 0.0f;
   float3 xs = make_float3(0.f, 0.f, 0.f);
-  nbrs(a, i, [&](uint32_t j) {
+  nbrs_split<L>(a, i, lq, [&](uint32_t j) {
     const float4 q = xw[j];
     const Pair p = pair_of(c, xi, f3(q));
     const float3 ebj = f3(etabar[j]);
     s = s - dot3(ebj, p.g);
     xs = xs + hess(p, q.w * ebi - wn * ebj);
   });
+  s = lane_sum<L>(s);
+  xs = lane_sum<L>(xs);
+  if (lq != 0) return;
   float3 wb = f3(wbar[i]);
   if (wn > 0.0f) wb = wb + (s / wn) * f3(w[i]);
   wbar[i] = make_float4(wb.x, wb.y, wb.z, 0.f);
@@ -214,6 +240,7 @@ __global__ __launch_bounds__(kB) void k_adj_vort_b(AdjDev a, SimConst c,
 
 // XSPH and vorticity: v2_bar and x*_bar; then v2 = (x*_S - x)/dt.
 // ybar = x*_S bar (all terms), xstart = -v2_bar/dt (the x part of v2).
+template <int L>
 __global__ __launch_bounds__(kB) void k_adj_xsph(AdjDev a, SimConst c,
                                                  const float4* __restrict__ xs,
                                                  const float4* __restrict__ v2,
@@ -223,15 +250,16 @@ __global__ __launch_bounds__(kB) void k_adj_xsph(AdjDev a, SimConst c,
                                                  float4* __restrict__ ybar,
                                                  float4* __restrict__ xstart,
                                                  uint32_t* __restrict__ contacts, SceneGrid g) {
-  const uint32_t i = idx();
+  const uint32_t i = idx() / L;
+  const int lq = (int)(idx() % L);
   if (i >= a.n) return;
   const float3 xi = f3(xs[i]);
   const float3 vi = f3(v2[i]);
   const float3 bi = f3(vb3[i]);
   const float3 wi = f3(wbar[i]);
-  float3 v2b = bi;
+  float3 v2b = make_float3(0.f, 0.f, 0.f);
   float3 xb = make_float3(0.f, 0.f, 0.f);
-  nbrs(a, i, [&](uint32_t j) {
+  nbrs_split<L>(a, i, lq, [&](uint32_t j) {
     const Pair p = pair_of(c, xi, f3(xs[j]));
     const float3 u = f3(v2[j]) - vi;
     const float3 bj = f3(vb3[j]);
@@ -239,6 +267,9 @@ __global__ __launch_bounds__(kB) void k_adj_xsph(AdjDev a, SimConst c,
     v2b = v2b + (c.xc * p.W) * (bj - bi) - cross3(p.g, ws);
     xb = xb + (c.xc * dot3(u, bi - bj)) * p.gP + hess(p, cross3(ws, u));
   });
+  v2b = bi + lane_sum<L>(v2b);
+  xb = lane_sum<L>(xb);
+  if (lq != 0) return;
   const float3 y = f3(xs_acc[i]) + xb + c.inv_dt * v2b;
   ybar[i] = make_float4(y.x, y.y, y.z, 0.f);
   const float3 xst = -c.inv_dt * v2b;
@@ -252,19 +283,21 @@ __global__ __launch_bounds__(kB) void k_adj_xsph(AdjDev a, SimConst c,
 // lambda_bar = sum_j g_ij . (e_i - e_j), e = ybar / rho0; then
 // C_bar = -lambda_bar/(D+eps), D_bar = -lambda_bar lambda/(D+eps),
 // rho_bar = C_bar/rho0.  aux = (lambda, rho_bar, D_bar, .), Gb = (G, .).
+template <int L>
 __global__ __launch_bounds__(kB) void k_adj_lambda(AdjDev a, SimConst c,
                                                    const float4* __restrict__ xs,
                                                    const float4* __restrict__ ybar,
                                                    float4* __restrict__ aux,
                                                    float4* __restrict__ Gb,
                                                    uint32_t* __restrict__ contacts, SceneGrid g) {
-  const uint32_t i = idx();
+  const uint32_t i = idx() / L;
+  const int lq = (int)(idx() % L);
   if (i >= a.n) return;
   const float3 xi = f3(xs[i]);
   const float3 ei = c.inv_rho0 * f3(ybar[i]);
   float rho = 0.f, D = 0.f, lb = 0.f;
   float3 G = make_float3(0.f, 0.f, 0.f);
-  nbrs(a, i, [&](uint32_t j) {
+  nbrs_split<L>(a, i, lq, [&](uint32_t j) {
     const Pair p = pair_of(c, xi, f3(xs[j]));
     rho = rho + p.W;
     const float3 gs = c.inv_rho0 * p.g;
@@ -272,6 +305,11 @@ __global__ __launch_bounds__(kB) void k_adj_lambda(AdjDev a, SimConst c,
     D = D + dot3(gs, gs);
     lb = lb + dot3(p.g, ei - c.inv_rho0 * f3(ybar[j]));
   });
+  rho = lane_sum<L>(rho);
+  G = lane_sum<L>(G);
+  D = lane_sum<L>(D);
+  lb = lane_sum<L>(lb);
+  if (lq != 0) return;
   D = D + dot3(G, G);
   const float C = rho / c.rho0 - 1.0f;
   const float lam = -C / (D + c.eps);
@@ -291,6 +329,7 @@ __global__ __launch_bounds__(kB) void k_adj_lambda(AdjDev a, SimConst c,
 //   + (rho_bar_i + rho_bar_j) gP                 (W in rho)
 //   + H [2 (D_bar_i + D_bar_j) g/rho0^2 + 2 (D_bar_i G_i - D_bar_j G_j)/rho0]  (g in D)
 // and dL/dk += sum_j (e_i . g_ij) (-(W S)^n).
+template <int L>
 __global__ __launch_bounds__(kB) void k_adj_pos(AdjDev a, SimConst c,
                                                 const float4* __restrict__ xs,
                                                 const float4* __restrict__ ybar,
@@ -298,7 +337,8 @@ __global__ __launch_bounds__(kB) void k_adj_pos(AdjDev a, SimConst c,
                                                 const float4* __restrict__ Gb,
                                                 float4* __restrict__ ynext,
                                                 double* __restrict__ kpart) {
-  const uint32_t i = idx();
+  const uint32_t i = idx() / L;
+  const int lq = (int)(idx() % L);
   if (i >= a.n) return;
   const float3 xi = f3(xs[i]);
   const float3 ei = c.inv_rho0 * f3(ybar[i]);
@@ -307,7 +347,7 @@ __global__ __launch_bounds__(kB) void k_adj_pos(AdjDev a, SimConst c,
   const float r02 = c.inv_rho0 * c.inv_rho0;
   float3 acc = make_float3(0.f, 0.f, 0.f);
   float kp = 0.f;
-  nbrs(a, i, [&](uint32_t j) {
+  nbrs_split<L>(a, i, lq, [&](uint32_t j) {
     const Pair p = pair_of(c, xi, f3(xs[j]));
     const float3 ej = c.inv_rho0 * f3(ybar[j]);
     const float4 aj = aux[j];
@@ -322,6 +362,9 @@ __global__ __launch_bounds__(kB) void k_adj_pos(AdjDev a, SimConst c,
     acc = acc + hess(p, hv) + (sd * dot3(de, p.g) + ai.y + aj.y) * p.gP;
     kp = kp + dot3(ei, p.g) * (-pw);
   });
+  acc = lane_sum<L>(acc);
+  kp = lane_sum<L>(kp);
+  if (lq != 0) return;
   const float3 y = f3(ybar[i]) + acc;
   ynext[i] = make_float4(y.x, y.y, y.z, 0.f);
   kpart[i] = (double)kp;
@@ -407,6 +450,7 @@ struct pbf_adj {
   float4 *etab = nullptr, *wb = nullptr, *aux = nullptr, *Gb = nullptr, *xbn = nullptr;
   double *kpart = nullptr, *ksteps = nullptr;
   double* kslice = nullptr;  // partial sums of the dL/dk reduction
+  int lanes = 1;             // lanes per particle in the adjoint neighbour passes
   uint32_t* contacts = nullptr;
   double backward_ms = 0.0;
   int64_t last_contacts = 0;
@@ -514,33 +558,35 @@ int forward_step(pbf_adj* a, const pbf::SimConst& k, AdjTape* slot) {
   return rc;
 }
 
-// Backward of step t, whose forward pass is recorded in tape slot T.
+// Backward of step t, whose forward pass is recorded in tape slot T, with L
+// lanes per particle in the neighbour passes.
 // xbar/vbar (load order) hold dL/d(state after the step) on entry and
 // dL/d(state before the step) on exit; ksteps[t] receives this step's dL/dk.
-int backward_step(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
+template <int L>
+int backward_step_l(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
   pbf::Ctx* c = X(a);
   const uint32_t n = (uint32_t)a->n;
   const size_t np = (size_t)a->npad;
   const AdjTape d = T;
   pbf::AdjDev ad{d.posB, d.nbr, d.ncnt, pbf::k4_of(*c), n};
-  const unsigned nb = pbf::nblk(n);
+  const unsigned nb = pbf::nblk(n), nbl = pbf::nblk((int64_t)n * L);
   hipStream_t s = c->stream;
   const pbf::SceneGrid g = c->grids[0];
   // dL/dv_new and dL/dx_new in sorted order.
   pbf::k_from_id<<<nb, pbf::kB, 0, s>>>(n, d.posB, a->vbar, a->vb3);
   pbf::k_from_id<<<nb, pbf::kB, 0, s>>>(n, d.posB, a->xbar, a->xbn);
-  pbf::k_adj_vort_a<<<nb, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->vb3, a->etab, a->wb);
-  pbf::k_adj_vort_b<<<nb, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->etab, a->wb, a->xbn,
-                                           a->xsacc);
-  pbf::k_adj_xsph<<<nb, pbf::kB, 0, s>>>(ad, k, d.predQ, d.velB, a->vb3, a->wb, a->xsacc, a->yb,
-                                         a->xst, a->contacts, g);
+  pbf::k_adj_vort_a<L><<<nbl, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->vb3, a->etab, a->wb);
+  pbf::k_adj_vort_b<L><<<nbl, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->etab, a->wb, a->xbn,
+                                              a->xsacc);
+  pbf::k_adj_xsph<L><<<nbl, pbf::kB, 0, s>>>(ad, k, d.predQ, d.velB, a->vb3, a->wb, a->xsacc,
+                                            a->yb, a->xst, a->contacts, g);
   AHIP(a, hipGetLastError());
   const int S = c->p.iterations;
   for (int it = S - 1; it >= 0; it--) {
     const float4* xs = d.stash + (size_t)it * np;
-    pbf::k_adj_lambda<<<nb, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->contacts, g);
-    pbf::k_adj_pos<<<nb, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->yb2,
-                                          a->kpart + (size_t)it * np);
+    pbf::k_adj_lambda<L><<<nbl, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->contacts, g);
+    pbf::k_adj_pos<L><<<nbl, pbf::kB, 0, s>>>(ad, k, xs, a->yb, a->aux, a->Gb, a->yb2,
+                                             a->kpart + (size_t)it * np);
     AHIP(a, hipGetLastError());
     std::swap(a->yb, a->yb2);
   }
@@ -551,6 +597,14 @@ int backward_step(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
   pbf::k_adj_predict<<<nb, pbf::kB, 0, s>>>(ad, k, a->yb, a->xst, a->xbar, a->vbar);
   AHIP(a, hipGetLastError());
   return PBF_OK;
+}
+
+int backward_step(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
+  switch (a->lanes) {
+    case 4: return backward_step_l<4>(a, k, t, T);
+    case 2: return backward_step_l<2>(a, k, t, T);
+    default: return backward_step_l<1>(a, k, t, T);
+  }
 }
 
 }  // namespace
@@ -572,6 +626,13 @@ PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32
   }
   a->n = n;
   a->npad = (n + 63) / 64 * 64;
+  // Four lanes per particle while that still leaves under ~4M threads
+  // (small scenes are latency-bound); PBF_ADJ_LANES = 1, 2 or 4 overrides.
+  a->lanes = n <= (1 << 20) ? 4 : 1;
+  if (const char* e = getenv("PBF_ADJ_LANES")) {
+    const int l = atoi(e);
+    if (l == 1 || l == 2 || l == 4) a->lanes = l;
+  }
   a->nsteps = nsteps;
   a->every = every;
   a->nckpt = (nsteps + every - 1) / every;

@@ -134,3 +134,33 @@ def test_torch_autograd_binding(run):
     g = run["vb0"]
     assert np.allclose(v0.grad.cpu().numpy(), g, rtol=1e-4, atol=1e-5 * np.abs(g).max())
     assert abs(float(k.grad) - run["kb"]) <= 1e-4 * abs(run["kb"])
+
+
+def test_lanes_per_particle_agree_and_repeat():
+    """The neighbour passes of the adjoint split each particle's list over
+    1, 2 or 4 lanes (PBF_ADJ_LANES; 4 is the default at this size).  The
+    splits add the same terms in other orders: they agree to rounding, and
+    each repeats its own bits."""
+    import os
+
+    sc, target = make_problem("amplified")
+    out = {}
+    for lanes in ("1", "2", "4", "4"):
+        os.environ["PBF_ADJ_LANES"] = lanes
+        try:
+            r = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=2)
+            xT, _ = r.forward(sc.x, sc.v, sc.params.k_tensile)
+            xbar = ((2.0 / sc.n) * (xT.astype(np.float64) - target)).astype(np.float32)
+            xb0, vb0, kb = r.backward(xbar)
+            r.close()
+        finally:
+            os.environ.pop("PBF_ADJ_LANES", None)
+        if lanes in out:
+            assert np.array_equal(out[lanes][0].view(np.uint32), vb0.view(np.uint32))
+            assert out[lanes][1] == kb
+        out[lanes] = (vb0, kb)
+    ref, kref = out["1"]
+    for lanes in ("2", "4"):
+        g, kb = out[lanes]
+        assert np.abs(g - ref).max() <= 1e-5 * np.abs(ref).max()
+        assert abs(kb - kref) <= 1e-5 * abs(kref)
