@@ -539,14 +539,18 @@ int forward_step(pbf_adj* a, const pbf::SimConst& k, AdjTape* slot) {
     if (slot) pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, slot->stash);
     for (int it = 0; it < c->p.iterations; it++) {
       const bool last = it == c->p.iterations - 1;
-      AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP));
+      AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP, it > 0));
       AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, last));
       // The last iteration writes x* and v as XSPH records; the adjoint reads
       // them from predQ and velB.
       if (last) AHIP(a, pbf::launch_unpack_record(*c, d.predQ, d.velB));
-      if (slot)
-        pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ,
-                                                             slot->stash + (size_t)(it + 1) * np);
+      if (slot) {
+        float4* st = slot->stash + (size_t)(it + 1) * np;
+        if (last)
+          pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, st);
+        else
+          AHIP(a, pbf::launch_unpack_xs3(*c, st));  // x* of a pass before the last
+      }
     }
     AHIP(a, pbf::launch_xsph(*c, k, d.predP));
     AHIP(a, pbf::launch_vort_commit(*c, k, d.predP));

@@ -84,7 +84,7 @@ void dfree(T*& p) {
 void free_device(Ctx* x) {
   pbf::DevState& d = x->d;
   dfree(d.posA); dfree(d.posB); dfree(d.velA); dfree(d.velB);
-  dfree(d.predP); dfree(d.predQ); dfree(d.vort); dfree(d.vxsph); dfree(d.xv);
+  dfree(d.predP); dfree(d.predQ); dfree(d.xs3); dfree(d.vort); dfree(d.vxsph); dfree(d.xv);
   dfree(d.cell); dfree(d.arrival); dfree(d.cellSorted); dfree(d.tmpId);
   dfree(d.cellCount); dfree(d.cellStart); dfree(d.blockSums);
   dfree(d.nbr); dfree(d.ncnt); dfree(d.flags);
@@ -118,6 +118,7 @@ int ensure_device(Ctx* x) {
   if (e == hipSuccess) e = dalloc(ptr, cnt)
   DA(d.posA, np); DA(d.posB, np); DA(d.velA, np); DA(d.velB, np);
   DA(d.predP, np + 64); DA(d.predQ, np + 64); DA(d.vort, np); DA(d.vxsph, np); DA(d.xv, 2 * np);
+  DA(d.xs3, 3 * (np + 64));
   DA(d.cell, np); DA(d.arrival, np); DA(d.cellSorted, np); DA(d.tmpId, np);
   DA(d.cellCount, C + 4); DA(d.cellStart, C + 4); DA(d.blockSums, nb);
   DA(d.nbr, np * K); DA(d.ncnt, np); DA(d.flags, 64);
@@ -284,7 +285,8 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr,
   x->has_nbrs = true;
   if (stage == PBF_STAGE_NEIGHBORS) return PBF_OK;
   for (int it = 0; it < x->p.iterations; it++) {
-    if (!(fuse && it == 0)) PASS(PBF_PASS_LAMBDA, pbf::launch_lambda(*x, c, cur, other));
+    if (!(fuse && it == 0))
+      PASS(PBF_PASS_LAMBDA, pbf::launch_lambda(*x, c, cur, other, it > 0));
     if (stage == PBF_STAGE_LAMBDA0 && it == 0) {
       x->dbg_pred = other;
       return PBF_OK;
@@ -294,7 +296,10 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr,
     // The last iteration leaves x* and v only in the XSPH records; a debug
     // stop unpacks them where the earlier stages keep them.
     if ((stage == PBF_STAGE_DELTAP0 && it == 0) || (stage == PBF_STAGE_SOLVE && last)) {
-      if (last) HIPCHK(x, pbf::launch_unpack_record(*x, cur, d.velB));
+      if (last)
+        HIPCHK(x, pbf::launch_unpack_record(*x, cur, d.velB));
+      else
+        HIPCHK(x, pbf::launch_unpack_xs3(*x, cur));
       x->dbg_pred = cur;
       return PBF_OK;
     }

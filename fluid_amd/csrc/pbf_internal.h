@@ -31,6 +31,7 @@ struct DevState {
   float4* velB = nullptr;   // sorted v for the current step
   float4* predP = nullptr;  // x* ping (xyz), lambda or |omega| (w)
   float4* predQ = nullptr;  // x* pong
+  float* xs3 = nullptr;     // x* packed as 3 floats (12-byte records), for the gathers
   float4* vort = nullptr;   // omega (xyz)
   float4* vxsph = nullptr;  // v after XSPH (xyz), density (w)
   float4* xv = nullptr;     // (x*, v) pairs after the solve, 2 per particle
@@ -106,7 +107,8 @@ hipError_t launch_scan(Ctx& x);
 // predictions are (NULL: predP).
 hipError_t launch_scatter_rank(Ctx& x, const float4* pred = nullptr);
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c);
-hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out);
+// packed: the x* records (xs3) hold in's xyz, as after a delta-p pass.
+hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out, bool packed);
 // Neighbour build and the first λ pass (in = predQ) in one launch.
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out);
 hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last);
@@ -114,6 +116,8 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
 hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out);
 // x* and v of the last iteration's records into float4 arrays.
 hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel);
+// x* of the 12-byte records (after a delta-p pass before the last) into pos.
+hipError_t launch_unpack_xs3(Ctx& x, float4* pos);
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in);
 // Vorticity and commit fused with the next step's prediction (zeroes the
 // histogram first); the next step then starts at the scan.

@@ -44,6 +44,10 @@ constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers a 24-byte record p
 // Device pointers and sizes of one context, passed by value to every kernel.
 struct Dev {
   float4 *posA, *posB, *velA, *velB, *predP, *predQ, *vort, *vxsph, *xv;
+  // x* as 12-byte records, written by the delta-p passes before the last:
+  // the λ passes after the first gather x* only, and 10.7 records share a
+  // 128-byte line instead of 8.
+  float3* xs3;
   uint32_t *cell, *arrival, *cellSorted, *tmpId, *cellCount, *cellStart, *nbr, *ncnt, *flags;
   const SceneGrid* grids;
   const uint32_t* slotScene;
@@ -57,6 +61,7 @@ Dev dev_of(const Ctx& x) {
   Dev d;
   d.posA = s.posA; d.posB = s.posB; d.velA = s.velA; d.velB = s.velB;
   d.predP = s.predP; d.predQ = s.predQ; d.vort = s.vort; d.vxsph = s.vxsph; d.xv = s.xv;
+  d.xs3 = reinterpret_cast<float3*>(s.xs3);
   d.cell = s.cell; d.arrival = s.arrival; d.cellSorted = s.cellSorted; d.tmpId = s.tmpId;
   d.cellCount = s.cellCount; d.cellStart = s.cellStart; d.nbr = s.nbr; d.ncnt = s.ncnt;
   d.flags = s.flags; d.grids = s.grids; d.slotScene = s.slotScene;
@@ -423,15 +428,24 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
 }
 
 // newtonStepCalculateLambda (src/particles.cpp:185-204).  out = (x*, lambda).
+// x* comes from `in` (after the sort), or with PACKED from the 12-byte
+// records that the previous delta-p pass wrote (then `in` is not read).
+template <bool PACKED>
 __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimConst& c,
                                            const float4* __restrict__ in,
                                            float4* __restrict__ out) {
-  const float4 xi = in[i];
+  const float3 xi = PACKED ? d.xs3[i] : make_float3(in[i].x, in[i].y, in[i].z);
   const int cnt = (int)d.ncnt[i];
   float dens = 0.0f, denom = 0.0f, Gx = 0.0f, Gy = 0.0f, Gz = 0.0f;
+  const float3* __restrict__ xs3 = d.xs3;
   visit_neighbors<kUnroll>(
-      i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
-      [&](uint32_t, const float4& xj) {
+      i, cnt, d.K4, d.nbr,
+      [&](uint32_t j) {
+        if (PACKED) return at(xs3, j);
+        const float4 q = at(in, j);
+        return make_float3(q.x, q.y, q.z);
+      },
+      [&](uint32_t, const float3& xj) {
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const float w = poly6_sel(c, r2);
@@ -486,7 +500,10 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   const float3 p =
       clamp_dp(g, c, d.mesh, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
   if (!LAST) {
-    out[i] = make_float4(p.x, p.y, p.z, 0.0f);
+    // The next λ pass reads x* from the 12-byte records only (this pass
+    // gathers from `in`); `out` is left as it is.
+    (void)out;
+    d.xs3[i] = p;
   } else {
     // The last iteration writes only the (x*, v) record that XSPH gathers.
     const float4 x = d.posB[i];
@@ -662,13 +679,14 @@ __global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneG
   const uint32_t i = gather_index();
   if (i >= d.n) return;
   nbr_one(i, d, c.h2, grid_of<MULTI>(i, g0, d));
-  lambda_one(i, d, c, d.predQ, out);
+  lambda_one<false>(i, d, c, d.predQ, out);
 }
 
+template <bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_lambda(Dev d, SimConst c, const float4* in,
                                                    float4* out) {
   const uint32_t i = gather_index();
-  if (i < d.n) lambda_one(i, d, c, in, out);
+  if (i < d.n) lambda_one<PACKED>(i, d, c, in, out);
 }
 
 template <bool MULTI, bool LAST>
@@ -693,6 +711,15 @@ __global__ __launch_bounds__(kBlock) void k_unpack_record(Dev d, float4* __restr
   const float3 x = r[0], v = r[1];
   pos[i] = make_float4(x.x, x.y, x.z, 0.0f);
   vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// Copies the 12-byte x* records into a float4 array (w = 0), for debug
+// stops and the adjoint's tape after a delta-p pass before the last.
+__global__ __launch_bounds__(kBlock) void k_unpack_xs3(Dev d, float4* __restrict__ pos) {
+  const uint32_t i = flat_index();
+  if (i >= d.n) return;
+  const float3 x = d.xs3[i];
+  pos[i] = make_float4(x.x, x.y, x.z, 0.0f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_vort_commit(Dev d, SimConst c, const float4* in) {
@@ -784,7 +811,13 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const S
       for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) nbr_one(i, d, c.h2, g);
       phase_barrier();
       for (int it = 0; it < iterations; it++) {
-        for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) lambda_one(i, d, c, d.predQ, d.predP);
+        if (it == 0) {
+          for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK)
+            lambda_one<false>(i, d, c, d.predQ, d.predP);
+        } else {
+          for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK)
+            lambda_one<true>(i, d, c, d.predQ, d.predP);
+        }
         phase_barrier();
         if (it == iterations - 1) {
           for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK)
@@ -959,8 +992,11 @@ hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   return hipGetLastError();
 }
 
-hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out) {
-  k_lambda<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out, bool packed) {
+  if (packed)
+    k_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+  else
+    k_lambda<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
   return hipGetLastError();
 }
 
@@ -983,6 +1019,11 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
 
 hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
   k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_xs3(Ctx& x, float4* pos) {
+  k_unpack_xs3<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), pos);
   return hipGetLastError();
 }
 
