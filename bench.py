@@ -14,10 +14,10 @@ HBM.  Prints ONE JSON line:
              launch (B/particle table below x N) / mean launch time, against
              8 TB/s.  `traffic` is the HBM bytes of one launch from the
              committed rocprofv3 counters (calibrated FETCH_SIZE +
-             WRITE_SIZE); `valu` is the issue roof of these passes:
-             wave-level VALU instructions of one launch (committed
-             counters) x 64 lanes / mean launch time, against 256 CU x 64
-             lanes x 2.4 GHz;
+             WRITE_SIZE); `valu` is the FP32 roof: wave-level VALU
+             instructions of one launch (committed counters) x 64 lanes /
+             mean launch time, against 157.3 TFLOP/s as 78.6e12 lane
+             operations per second;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
@@ -36,7 +36,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # unpacked FP32/INT32 lane operations per second
+# MI355X's published FP32 peak, 157.3 TFLOP/s, as lane operations (an FMA
+# counts once): 256 CUs x 128 lanes x 2.4 GHz.
+VALU_PEAK_LANE_OPS = 256 * 128 * 2.4e9
 
 # Algorithmic FP32 state bytes per particle of every pass (reads + writes of
 # the particle state the pass needs; neighbour gathers assumed on-chip).
