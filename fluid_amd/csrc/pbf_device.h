@@ -365,6 +365,24 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t i, int k, int K4) {
   return nbr_chunk(i, k >> 2, K4) + (size_t)(k & 3);
 }
 
+// One 4-entry list chunk.  Lists are read once per pass, so the load is
+// non-temporal: it does not displace gathered neighbour state in the L1
+// (λ -4%, Δp -4%, build + first λ -3%; the grouped visit of XSPH gained
+// nothing and keeps plain loads).
+#ifndef PBF_NT_INDEX
+#define PBF_NT_INDEX 1
+#endif
+__device__ __forceinline__ uint4 load_chunk(const uint32_t* __restrict__ p) {
+#if PBF_NT_INDEX
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 q = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+  return make_uint4(q.x, q.y, q.z, q.w);
+#else
+  return *reinterpret_cast<const uint4*>(p);
+#endif
+}
+
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Remapping the hardware block index so that every XCD runs a
 // contiguous range of logical blocks keeps the cell-sorted neighbourhoods a
@@ -501,10 +519,7 @@ __device__ __forceinline__ void for_each_neighbor_pipe(uint32_t i, int cnt, int 
                                                        const uint32_t* __restrict__ nbr,
                                                        Load load, Body body) {
   using D = decltype(load(0u));
-  auto idx = [&](int ch) {
-    const uint4 q = *reinterpret_cast<const uint4*>(nbr + nbr_chunk(i, ch, K4));
-    return q;
-  };
+  auto idx = [&](int ch) { return load_chunk(nbr + nbr_chunk(i, ch, K4)); };
   auto gather = [&](const uint4& q, D* d) {
     d[0] = load(q.x);
     d[1] = load(q.y);
