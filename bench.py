@@ -37,7 +37,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # MI355X's published FP32 peak, 157.3 TFLOP/s, as lane operations (an FMA
-# counts once): 256 CUs x 128 lanes x 2.4 GHz.
+# counts once): 256 CUs x 128 lanes x 2.4 GHz.  The figure counts packed FP32
+# (two operations per lane and instruction); code built without packed FP32,
+# as this library is, issues one wave64 VALU instruction per 4 cycles per
+# SIMD, half of it.
 VALU_PEAK_LANE_OPS = 256 * 128 * 2.4e9
 
 # Algorithmic FP32 state bytes per particle of every pass (reads + writes of
