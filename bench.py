@@ -19,6 +19,9 @@ HBM.  Prints ONE JSON line:
              mean launch time, against 157.3 TFLOP/s as 78.6e12 lane
              operations per second;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
+* density_field, snapshot, mesh, batched, adjoint  the paths beside the step
+             (marching-cubes density lattice, checkpoint files, the
+             triangle-mesh collider, many small scenes, gradients);
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
 
@@ -191,6 +194,85 @@ def density_field_bench(ctx, sc):
     }
 
 
+def snapshot_bench(ctx, sc):
+    """Binary snapshot of the committed state (pbf_save_snapshot) written to
+    a temporary file and loaded back (pbf_load_snapshot, checksum verified);
+    the state read after the load must equal the state before, bit for bit."""
+    import tempfile
+
+    import numpy as np
+
+    before = ctx.read()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "state.pbfsnap")
+        t0 = time.perf_counter()
+        ctx.save_snapshot(path)
+        save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        ctx.load_snapshot(path)
+        load = time.perf_counter() - t0
+    after = ctx.read()
+    exact = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(before, after))
+    log(f"snapshot {size / 1e6:.1f} MB: save {save * 1e3:.1f} ms, load {load * 1e3:.1f} ms, "
+        f"round trip bit-exact {exact}")
+    return {
+        "metric": "snapshot MB/s",
+        "bytes": size,
+        "save_ms": save * 1e3,
+        "load_ms": load * 1e3,
+        "save_MBs": size / save / 1e6,
+        "load_MBs": size / load / 1e6,
+        "round_trip_bit_exact": exact,
+    }
+
+
+def mesh_bench(sc, warmup: int, steps: int, plain_ms: float):
+    """The same scene with the mesh collider: the box as 12 triangles plus an
+    icosphere obstacle (5,120 triangles, radius 0.45) between the two
+    spheres, which the fluid reaches within the first steps."""
+    import numpy as np
+
+    from fluid_amd import mesh
+    from fluid_amd.pbf import Particles
+
+    p = sc.params
+    m = mesh.merge(mesh.box(p.box_lo, p.box_hi), mesh.icosphere((2.5, 1.5, 1.5), 0.45, 4))
+    skin = mesh.default_skin(m[0], p.box_lo, p.box_hi)
+    ctx = Particles(p)
+    ctx.set_mesh(*m, skin=skin)
+    ctx.load(sc.x, sc.v)
+    ctx.time_step(nsteps=warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.time_step(nsteps=steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    x, _, _ = ctx.read()
+    ctx.close()
+    # Particles on the obstacle's surface (within 2 h) and inside its
+    # inscribed sphere (the icosphere's faces lie inside the circumsphere).
+    r = np.linalg.norm(x.astype(np.float64) - np.array([2.5, 1.5, 1.5]), axis=1)
+    on_surface = int((r < 0.45 + 2 * p.h).sum())
+    inside = int((r < 0.45 * 0.98).sum())
+    ms = wall / steps * 1e3
+    log(f"mesh collider ({len(m[2])} triangles): {steps} steps, {ms:.3f} ms/step "
+        f"(box walls {plain_ms:.3f}); {on_surface} particles within 2h of the obstacle, "
+        f"{inside} inside")
+    return {
+        "metric": "particle-steps/s with the mesh collider",
+        "triangles": int(len(m[2])),
+        "obstacle": "icosphere r=0.45 at (2.5, 1.5, 1.5), 5120 triangles, plus the box walls",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms,
+        "particle_steps_per_s": sc.n * steps / wall,
+        "vs_box_walls": plain_ms / ms,
+        "particles_near_obstacle": on_surface,
+        "particles_inside_obstacle": inside,
+    }
+
+
 def adjoint_bench():
     """32^3 block, 60 steps, checkpoints every 10: wall time of the backward
     pass (recompute included) per step, and of the forward pass."""
@@ -237,6 +319,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--looped-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
@@ -283,6 +366,7 @@ def main() -> int:
     log(f"profiled {args.steps} steps: {wall_profiled:.3f} s")
     nbar, nmax = ctx.neighbor_stats()
     field = density_field_bench(ctx, sc)
+    snap = snapshot_bench(ctx, sc)
     ctx.close()
     ps = args.steps
     ncell = 1
@@ -366,7 +450,10 @@ def main() -> int:
         "ms_per_step_profiled": wall_profiled / args.steps * 1e3,
         "passes": passes,
         "density_field": field,
+        "snapshot": snap,
     }
+    if not args.no_mesh:
+        out["mesh"] = mesh_bench(sc, args.warmup, 10, wall / args.steps * 1e3)
     if not args.no_batched:
         out["batched"] = batched_bench(args.batched_steps, args.looped_steps)
     if not args.no_adjoint:

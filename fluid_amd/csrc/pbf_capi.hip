@@ -489,6 +489,7 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   free_device(x);
   dfree(x->d.meshTri);
   dfree(x->d.meshNode);
+  dfree(x->d.meshFlag);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return PBF_OK;
@@ -859,6 +860,7 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
   drop_graph(x);
   dfree(x->d.meshTri);
   dfree(x->d.meshNode);
+  dfree(x->d.meshFlag);
   x->mesh_ntri = 0;
   if (nt == 0) return PBF_OK;
   MeshBuild mb;
@@ -902,8 +904,52 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
       tv[6 * i + 3 + j] = make_float4(q[0], q[1], q[2], 0.0f);
     }
   }
+  // Cells of the scene grid near a triangle: a segment whose box (grown by
+  // pad) touches none of them skips the BVH walk (mesh_may_hit).  Triangle
+  // boxes are grown by 3 pad here (a hit point lies within pad/2 of its
+  // triangle's box, hit_in_box), far more than the float cell indexing on
+  // the device can err; if marking would visit more than 64 cells per grid
+  // cell, every cell is flagged (the walk always runs).
+  // PBF_MESH_CELLS=0 leaves the cells out (every segment walks the BVH).
+  const pbf::SceneGrid& g = x->grids[0];
+  const char* cells_env = getenv("PBF_MESH_CELLS");
+  const bool use_cells = !(cells_env && cells_env[0] == '0');
+  const size_t ncell = use_cells ? (size_t)g.dims[0] * (size_t)g.dims[1] * (size_t)g.dims[2] : 0;
+  std::vector<uint8_t> flag(ncell, 0);
+  if (use_cells) {
+    size_t work = 0;
+    bool all = false;
+    auto cell = [&](double v, int a) {
+      const double f = std::floor((v - (double)g.lo[a]) * (double)g.inv_cell);
+      if (!(f >= 0.0)) return 0;
+      if (!(f < (double)g.dims[a])) return g.dims[a] - 1;
+      return (int)f;
+    };
+    for (size_t k = 0; k < n && !all; k++) {
+      int c0[3], c1[3];
+      size_t cells = 1;
+      for (int a = 0; a < 3; a++) {
+        c0[a] = cell((double)mb.lo[3 * k + a] - 3.0 * mb.pad, a);
+        c1[a] = cell((double)mb.hi[3 * k + a] + 3.0 * mb.pad, a);
+        cells *= (size_t)(c1[a] - c0[a] + 1);
+      }
+      work += cells;
+      if (work > 64 * ncell) {
+        all = true;
+        break;
+      }
+      for (int z = c0[2]; z <= c1[2]; z++)
+        for (int y = c0[1]; y <= c1[1]; y++)
+          for (int xx = c0[0]; xx <= c1[0]; xx++)
+            flag[((size_t)z * g.dims[1] + y) * g.dims[0] + xx] = 1;
+    }
+    if (all) std::fill(flag.begin(), flag.end(), (uint8_t)1);
+  }
   hipError_t e = dalloc(x->d.meshTri, tv.size());
   if (e == hipSuccess) e = dalloc(x->d.meshNode, mb.nodes.size());
+  if (e == hipSuccess && use_cells) e = dalloc(x->d.meshFlag, ncell);
+  if (e == hipSuccess && use_cells)
+    e = hipMemcpy(x->d.meshFlag, flag.data(), ncell, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(x->d.meshTri, tv.data(), tv.size() * sizeof(float4), hipMemcpyHostToDevice);
   if (e == hipSuccess)
@@ -912,10 +958,12 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
   if (e != hipSuccess) {
     dfree(x->d.meshTri);
     dfree(x->d.meshNode);
+    dfree(x->d.meshFlag);
     return fail(x, PBF_ERR_NOMEM, std::string("mesh upload: ") + hipGetErrorString(e));
   }
   x->mesh_ntri = (uint32_t)nt;
   x->mesh_skin = skin;
+  x->mesh_margin = mb.pad;
   return PBF_OK;
 }
 

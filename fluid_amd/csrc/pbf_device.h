@@ -90,12 +90,38 @@ struct MeshView {
   const float4* node = nullptr;
   uint32_t ntri = 0;  // 0: the analytic box walls
   float skin = 0.0f;  // clearance a hit leaves along the face normal
+  // One byte per cell of the scene grid: nonzero where a BVH leaf box, grown
+  // by 2 margin, overlaps the cell (NULL: always walk the BVH).
+  const uint8_t* flag = nullptr;
+  float margin = 0.0f;
 };
 
 __device__ __forceinline__ float3 f3(const float4& a) { return make_float3(a.x, a.y, a.z); }
 __device__ __forceinline__ float dot3(float3 u, float3 v) { return u.x * v.x + u.y * v.y + u.z * v.z; }
 __device__ __forceinline__ float3 cross3(float3 u, float3 v) {
   return make_float3(u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x);
+}
+
+// Whether the hit point o + t d lies within hp of the box of the triangle
+// (p, p + e1, p + e2).  Every exact hit does; the test rejects a rounding
+// accept far from the triangle (a near-zero determinant), so that a walk
+// that skips far triangles (the BVH's padded boxes, the flagged cells)
+// finds what a scan of every triangle finds.
+__device__ __forceinline__ bool hit_in_box(float3 o, float3 d, float t, const float4& p,
+                                           float3 e1, float3 e2, float hp) {
+  const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+  const float pa[3] = {p.x, p.y, p.z}, a1[3] = {e1.x, e1.y, e1.z}, a2[3] = {e2.x, e2.y, e2.z};
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float q = oa[a] + t * da[a];
+    const float v1 = pa[a] + a1[a], v2 = pa[a] + a2[a];
+    float lo = pa[a] < v1 ? pa[a] : v1, hi = pa[a] > v1 ? pa[a] : v1;
+    lo = v2 < lo ? v2 : lo;
+    hi = v2 > hi ? v2 : hi;
+    in = in && (q >= lo - hp && q <= hi + hp);
+  }
+  return in;
 }
 
 // Nearest triangle hit of the segment o + t d, 0 <= t <= tmax: the smallest
@@ -156,6 +182,7 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
         const float v = dot3(s2, d) / det;
         const float w = (1.0f - u) - v;
         if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || w < 0.0f || w > 1.0f) continue;
+        if (!hit_in_box(o, d, t, P, e1, e2, m.margin * 0.5f)) continue;
         const float3 n1 = f3(m.tri[6 * (size_t)k + 3]), n2 = f3(m.tri[6 * (size_t)k + 4]),
                      n3 = f3(m.tri[6 * (size_t)k + 5]);
         bt = t;
@@ -171,6 +198,37 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
   }
   *t_out = bt;
   return best;
+}
+
+// Whether the segment o + t d, 0 <= t <= l, can hit the mesh.  A hit point
+// lies within margin/2 of its triangle's box (hit_in_box); the flagged
+// cells cover every triangle box grown by 3 margins, so a segment whose box,
+// grown by one margin, touches no flagged cell holds no such point (the
+// float cell indexing errs far less than a margin).  Segments that span
+// more than two cells on an axis, and non-finite ones, walk the BVH.
+__device__ __forceinline__ bool mesh_may_hit(const MeshView& m, const SceneGrid& g, float3 o,
+                                             float3 d, float l) {
+  if (!m.flag) return true;
+  const float s[3] = {o.x, o.y, o.z};
+  const float e[3] = {o.x + d.x * l, o.y + d.y * l, o.z + d.z * l};
+  int c0[3], c1[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float lo = fminf(s[a], e[a]) - m.margin, hi = fmaxf(s[a], e[a]) + m.margin;
+    if (!(fabsf(lo) < 3.0e38f) || !(fabsf(hi) < 3.0e38f) || !(fabsf(s[a]) < 3.0e38f) ||
+        !(fabsf(e[a]) < 3.0e38f))
+      return true;
+    const float f0 = floorf((lo - g.lo[a]) * g.inv_cell), f1 = floorf((hi - g.lo[a]) * g.inv_cell);
+    c0[a] = f0 < 0.0f ? 0 : (f0 < (float)g.dims[a] ? (int)f0 : g.dims[a] - 1);
+    c1[a] = f1 < 0.0f ? 0 : (f1 < (float)g.dims[a] ? (int)f1 : g.dims[a] - 1);
+    if (c1[a] - c0[a] > 1) return true;
+  }
+  for (int z = c0[2]; z <= c1[2]; z++)
+    for (int y = c0[1]; y <= c1[1]; y++)
+      for (int x = c0[0]; x <= c1[0]; x++)
+        if (m.flag[((size_t)z * (size_t)g.dims[1] + (size_t)y) * (size_t)g.dims[0] + (size_t)x])
+          return true;
+  return false;
 }
 
 // Box collision along dir for a displacement D of length l (= total).
@@ -225,7 +283,7 @@ __device__ inline float3 collide_mesh(const SceneGrid& g, const SimConst& c, con
   const bool intersect = stop_planes(g, pa, da, &l);
   float th;
   float3 n = make_float3(0.0f, 0.0f, 0.0f), nf = n;
-  const int hit = mesh_nearest(m, p, dir, l, &th, &n, &nf);
+  const int hit = mesh_may_hit(m, g, p, dir, l) ? mesh_nearest(m, p, dir, l, &th, &n, &nf) : -1;
   if (hit >= 0) l = th;
   if (hit >= 0 || intersect) {
     const float s = l - c.eps_d;
@@ -242,7 +300,9 @@ __device__ inline float3 collide_mesh(const SceneGrid& g, const SimConst& c, con
         const float3 d2 = make_float3(rn * T[0], rn * T[1], rn * T[2]);
         float l2 = (total - l) * 0.5f, t2;
         float3 n2, nf2;
-        const int hit2 = mesh_nearest(m, make_float3(pa[0], pa[1], pa[2]), d2, l2, &t2, &n2, &nf2);
+        const float3 p2 = make_float3(pa[0], pa[1], pa[2]);
+        const int hit2 =
+            mesh_may_hit(m, g, p2, d2, l2) ? mesh_nearest(m, p2, d2, l2, &t2, &n2, &nf2) : -1;
         if (hit2 >= 0) l2 = t2;
         const float s2 = l2 - c.eps_d;
         pa[0] = pa[0] + s2 * d2.x;

@@ -51,6 +51,7 @@ struct DevState {
   float* stage = nullptr;         // read-back staging, n * 4 floats
   float4* meshTri = nullptr;      // collider triangles, 6 float4 each (pbf_device.h MeshView)
   float4* meshNode = nullptr;     // collider BVH, 2 float4 per node
+  uint8_t* meshFlag = nullptr;    // scene-grid cells near a BVH leaf box (MeshView::flag)
 };
 
 struct Ctx {
@@ -75,6 +76,7 @@ struct Ctx {
   bool field_ready = false;          // committed state sorted by cell, cell table current
   uint32_t mesh_ntri = 0;            // triangles of the mesh collider (0: box walls)
   float mesh_skin = 0.0f;            // clearance a mesh hit leaves along the face normal
+  float mesh_margin = 0.0f;          // MeshView::margin
   const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop
   const float4* dbg_vel = nullptr;   // velocity buffer current at the last debug stop
   int schedule = PBF_SCHED_FLAT;     // PBF_SCHED_*

@@ -69,6 +69,8 @@ Dev dev_of(const Ctx& x) {
   d.mesh.node = s.meshNode;
   d.mesh.ntri = x.mesh_ntri;
   d.mesh.skin = x.mesh_skin;
+  d.mesh.flag = s.meshFlag;
+  d.mesh.margin = x.mesh_margin;
   d.n = (uint32_t)x.n;
   d.K4 = k4_of(x);
   return d;

@@ -20,6 +20,7 @@ typedef struct {
   int iters;
   unsigned stop;
   const orc_params *P; /* for the mesh collider */
+  REAL mesh_hp;        /* half the mesh padding: a hit point must lie this close to the triangle's box */
 } DER;
 
 /* poly6_kernel, src/particles.cpp:134-141:
@@ -95,6 +96,25 @@ static void FN(der_init_)(const orc_params *p, REAL dt, DER *d) {
   d->n_tensile = p->n_tensile;
   d->iters = p->iterations;
   d->stop = p->wall_stop_mask;
+  /* Mesh padding: 1e-5 of the largest coordinate of the triangles' vertices
+   * and the box (as pbf_set_mesh computes it for the BVH boxes). */
+  d->mesh_hp = (REAL)0;
+  if (p->mesh_nt > 0) {
+    REAL scale = (REAL)0;
+    for (int64_t k = 0; k < 3 * p->mesh_nt; k++)
+      for (int a = 0; a < 3; a++) {
+        REAL q = (REAL)p->mesh_pos[3 * (size_t)p->mesh_tri[k] + a];
+        q = q < (REAL)0 ? -q : q;
+        scale = q > scale ? q : scale;
+      }
+    for (int a = 0; a < 3; a++) {
+      REAL l = d->lo[a] < (REAL)0 ? -d->lo[a] : d->lo[a];
+      REAL u = d->hi[a] < (REAL)0 ? -d->hi[a] : d->hi[a];
+      scale = l > scale ? l : scale;
+      scale = u > scale ? u : scale;
+    }
+    d->mesh_hp = ((REAL)1e-5f * (scale > (REAL)1e-6f ? scale : (REAL)1e-6f)) * (REAL)0.5;
+  }
 }
 
 void FN(orc_derive_)(const orc_params *p, REAL *out, int *dims) {
@@ -168,6 +188,19 @@ static int FN(mesh_nearest_)(const DER *d, const REAL *o, const REAL *dir, REAL 
     REAL w = ((REAL)1 - u) - v;
     if (u < (REAL)0 || u > (REAL)1 || v < (REAL)0 || v > (REAL)1 || w < (REAL)0 || w > (REAL)1)
       continue;
+    /* The hit point must lie within mesh_hp of the triangle's box: true of
+     * every exact hit; it rejects a rounding accept far from the triangle
+     * (det near 0), so that a BVH that skips far triangles finds the same. */
+    int inbox = 1;
+    for (int a = 0; a < 3; a++) {
+      REAL q = o[a] + t * dir[a];
+      REAL v1 = p1[a] + e1[a], v2 = p1[a] + e2[a];
+      REAL lo = p1[a] < v1 ? p1[a] : v1, hi = p1[a] > v1 ? p1[a] : v1;
+      lo = v2 < lo ? v2 : lo;
+      hi = v2 > hi ? v2 : hi;
+      if (!(q >= lo - d->mesh_hp && q <= hi + d->mesh_hp)) inbox = 0;
+    }
+    if (!inbox) continue;
     bt = t;
     best = (int)k;
     FN(cross3_)(e1, e2, nf_out);

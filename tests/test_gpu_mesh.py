@@ -95,6 +95,36 @@ def test_box_mesh_matches_box_walls_and_removal_restores_them():
     assert bits_equal(cx, ax) and bits_equal(cv, av)
 
 
+def test_mesh_cell_pruning_changes_no_bit(monkeypatch):
+    """Segments whose box touches no grid cell near a BVH leaf skip the walk;
+    with the cells switched off (PBF_MESH_CELLS=0) every segment walks it.
+    Random velocities up to 3 m/s (0.3 cells per step) and a few particles
+    at 30 m/s (segments over two cells long) hit the sphere and the walls."""
+    p, x, _, m = drop_scene()
+    rng = np.random.default_rng(11)
+    v = rng.uniform(-3.0, 3.0, size=x.shape).astype(np.float32)
+    v[::37] *= 10.0
+
+    def run(cells):
+        with monkeypatch.context() as mp:
+            if not cells:
+                mp.setenv("PBF_MESH_CELLS", "0")
+            with Particles(p) as c:
+                c.set_mesh(*m)
+                c.load(x, v)
+                out = []
+                for _ in range(4):
+                    c.time_step(nsteps=10)
+                    out.append(c.read())
+                return out
+
+    walked, pruned = run(False), run(True)
+    for a, b in zip(walked, pruned):
+        assert all(bits_equal(u, w) for u, w in zip(a, b))
+    r = np.linalg.norm(pruned[-1][0] - np.array([0.5, 0.3, 0.5]), axis=1)
+    assert r.min() > 0.18
+
+
 def test_mesh_argument_errors():
     sc = scenes.dam_block()
     pos, nrm, tri = mesh.icosphere((0.5, 0.5, 0.5), 0.1, 1)
