@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -615,8 +617,8 @@ int backward_step(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T) {
 
 extern "C" {
 
-PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32_t every,
-                           pbf_adj** out) {
+static int pbf_adj_create_impl(const pbf_params* p, int64_t n, int32_t nsteps, int32_t every,
+                               pbf_adj** out) {
   if (!out) return afail(nullptr, PBF_ERR_INVALID, "out is NULL");
   *out = nullptr;
   if (n < 1 || nsteps < 1 || every < 1)
@@ -672,6 +674,17 @@ PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32
   return PBF_OK;
 }
 
+PBF_API int pbf_adj_create(const pbf_params* p, int64_t n, int32_t nsteps, int32_t every,
+                           pbf_adj** out) {
+  try {
+    return pbf_adj_create_impl(p, n, nsteps, every, out);
+  } catch (const std::bad_alloc&) {
+    return afail(nullptr, PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return afail(nullptr, PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
 PBF_API int pbf_adj_destroy(pbf_adj* a) {
   if (!a) return PBF_OK;
   pbf::Ctx* c = X(a);
@@ -687,8 +700,8 @@ PBF_API const char* pbf_adj_last_error(const pbf_adj* a) {
   return g_adj_err.c_str();
 }
 
-PBF_API int pbf_adj_forward(pbf_adj* a, const float* x0, const float* v0, float k_tensile,
-                            float* xT, float* vT) {
+static int pbf_adj_forward_impl(pbf_adj* a, const float* x0, const float* v0, float k_tensile,
+                                float* xT, float* vT) {
   if (!a || !x0 || !v0) return afail(a, PBF_ERR_INVALID, "NULL argument");
   pbf::Ctx* c = X(a);
   c->p.k_tensile = k_tensile;
@@ -713,8 +726,19 @@ PBF_API int pbf_adj_forward(pbf_adj* a, const float* x0, const float* v0, float 
   return PBF_OK;
 }
 
-PBF_API int pbf_adj_backward(pbf_adj* a, const float* xbar_T, const float* vbar_T, float* xbar0,
-                             float* vbar0, double* kbar) {
+PBF_API int pbf_adj_forward(pbf_adj* a, const float* x0, const float* v0, float k_tensile,
+                            float* xT, float* vT) {
+  try {
+    return pbf_adj_forward_impl(a, x0, v0, k_tensile, xT, vT);
+  } catch (const std::bad_alloc&) {
+    return afail(a, PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return afail(a, PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_adj_backward_impl(pbf_adj* a, const float* xbar_T, const float* vbar_T, float* xbar0,
+                                 float* vbar0, double* kbar) {
   if (!a || !xbar_T) return afail(a, PBF_ERR_INVALID, "NULL argument");
   if (!a->have_forward) return afail(a, PBF_ERR_STATE, "run pbf_adj_forward first");
   pbf::Ctx* c = X(a);
@@ -774,6 +798,17 @@ PBF_API int pbf_adj_backward(pbf_adj* a, const float* xbar_T, const float* vbar_
   }
   if (kbar) *kbar = kb;
   return PBF_OK;
+}
+
+PBF_API int pbf_adj_backward(pbf_adj* a, const float* xbar_T, const float* vbar_T, float* xbar0,
+                             float* vbar0, double* kbar) {
+  try {
+    return pbf_adj_backward_impl(a, xbar_T, vbar_T, xbar0, vbar0, kbar);
+  } catch (const std::bad_alloc&) {
+    return afail(a, PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return afail(a, PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 PBF_API int pbf_adj_stats(const pbf_adj* a, double* backward_ms, int64_t* wall_contacts) {

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -430,7 +432,7 @@ PBF_API void pbf_params_default(pbf_params* p) {
   p->eps_d = 0.00000000001f;
 }
 
-PBF_API int pbf_create(const pbf_params* p, pbf_ctx** out) {
+static int pbf_create_impl(const pbf_params* p, pbf_ctx** out) {
   std::string why;
   if (!out) return fail(nullptr, PBF_ERR_INVALID, "out is NULL");
   *out = nullptr;
@@ -447,8 +449,18 @@ PBF_API int pbf_create(const pbf_params* p, pbf_ctx** out) {
   return finish_create(x, out);
 }
 
-PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t* counts,
-                             const float* box_lo, const float* box_hi, pbf_ctx** out) {
+PBF_API int pbf_create(const pbf_params* p, pbf_ctx** out) {
+  try {
+    return pbf_create_impl(p, out);
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(nullptr, PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_create_batch_impl(const pbf_params* p, int32_t nscenes, const int64_t* counts,
+                                 const float* box_lo, const float* box_hi, pbf_ctx** out) {
   std::string why;
   if (!out) return fail(nullptr, PBF_ERR_INVALID, "out is NULL");
   *out = nullptr;
@@ -480,6 +492,17 @@ PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t
   return finish_create(x, out);
 }
 
+PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t* counts,
+                             const float* box_lo, const float* box_hi, pbf_ctx** out) {
+  try {
+    return pbf_create_batch_impl(p, nscenes, counts, box_lo, box_hi, out);
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(nullptr, PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
 PBF_API int pbf_destroy(pbf_ctx* c) {
   Ctx* x = C(c);
   if (!x) return PBF_OK;
@@ -499,10 +522,20 @@ namespace {
 int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_t n);
 }
 
-PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
+static int pbf_load_impl(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   return load_impl(x, xyz, v, nullptr, n);
+}
+
+PBF_API int pbf_load(pbf_ctx* c, const float* xyz, const float* v, int64_t n) {
+  try {
+    return pbf_load_impl(c, xyz, v, n);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 namespace {
@@ -659,11 +692,21 @@ int check_step_args(Ctx* x, float dt, int32_t nsteps) {
 
 }  // namespace
 
-PBF_API int pbf_step_async(pbf_ctx* c, float dt, int32_t nsteps) {
+static int pbf_step_async_impl(pbf_ctx* c, float dt, int32_t nsteps) {
   Ctx* x = C(c);
   int rc = check_step_args(x, dt, nsteps);
   if (rc != PBF_OK) return rc;
   return enqueue_steps(x, dt, nsteps);
+}
+
+PBF_API int pbf_step_async(pbf_ctx* c, float dt, int32_t nsteps) {
+  try {
+    return pbf_step_async_impl(c, dt, nsteps);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 PBF_API int pbf_sync(pbf_ctx* c) {
@@ -673,7 +716,7 @@ PBF_API int pbf_sync(pbf_ctx* c) {
   return sync_check(x);
 }
 
-PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
+static int pbf_step_dt_impl(pbf_ctx* c, float dt, int32_t nsteps) {
   Ctx* x = C(c);
   int rc = check_step_args(x, dt, nsteps);
   if (rc != PBF_OK) return rc;
@@ -681,7 +724,17 @@ PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
   return sync_check(x);
 }
 
-PBF_API int pbf_set_schedule(pbf_ctx* c, int32_t schedule, int32_t block) {
+PBF_API int pbf_step_dt(pbf_ctx* c, float dt, int32_t nsteps) {
+  try {
+    return pbf_step_dt_impl(c, dt, nsteps);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_set_schedule_impl(pbf_ctx* c, int32_t schedule, int32_t block) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   if (schedule < PBF_SCHED_FLAT || schedule > PBF_SCHED_PERSISTENT)
@@ -694,8 +747,18 @@ PBF_API int pbf_set_schedule(pbf_ctx* c, int32_t schedule, int32_t block) {
   return PBF_OK;
 }
 
-PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_ms,
-                              int32_t* pass_launches, int32_t npass) {
+PBF_API int pbf_set_schedule(pbf_ctx* c, int32_t schedule, int32_t block) {
+  try {
+    return pbf_set_schedule_impl(c, schedule, block);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_step_profiled_impl(pbf_ctx* c, float dt, int32_t nsteps, float* pass_ms,
+                                  int32_t* pass_launches, int32_t npass) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -732,13 +795,34 @@ PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_
   return sync_check(x);
 }
 
-PBF_API int pbf_step(pbf_ctx* c, int32_t nsteps) {
+PBF_API int pbf_step_profiled(pbf_ctx* c, float dt, int32_t nsteps, float* pass_ms,
+                              int32_t* pass_launches, int32_t npass) {
+  try {
+    return pbf_step_profiled_impl(c, dt, nsteps, pass_ms, pass_launches, npass);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_step_impl(pbf_ctx* c, int32_t nsteps) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   return pbf_step_dt(c, x->p.dt, nsteps);
 }
 
-PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
+PBF_API int pbf_step(pbf_ctx* c, int32_t nsteps) {
+  try {
+    return pbf_step_impl(c, nsteps);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_step_debug_impl(pbf_ctx* c, float dt, int32_t stage) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -756,7 +840,17 @@ PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
   return PBF_OK;
 }
 
-PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
+PBF_API int pbf_step_debug(pbf_ctx* c, float dt, int32_t stage) {
+  try {
+    return pbf_step_debug_impl(c, dt, stage);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_read_impl(pbf_ctx* c, float* xyz, float* v, float* rho) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -775,6 +869,16 @@ PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
     if ((rc = copy_out(x, d.stage, rho, n * sizeof(float))) != PBF_OK) return rc;
   }
   return PBF_OK;
+}
+
+PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
+  try {
+    return pbf_read_impl(c, xyz, v, rho);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 }  // extern "C"
@@ -839,8 +943,8 @@ struct MeshBuild {
 
 extern "C" {
 
-PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t nv,
-                         const int32_t* tri, int64_t nt, float skin) {
+static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int64_t nv,
+                             const int32_t* tri, int64_t nt, float skin) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   if (!is_finite_f(skin) || skin < 0.0f) return fail(x, PBF_ERR_INVALID, "skin must be >= 0");
@@ -967,6 +1071,17 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
   return PBF_OK;
 }
 
+PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t nv,
+                         const int32_t* tri, int64_t nt, float skin) {
+  try {
+    return pbf_set_mesh_impl(c, pos, nrm, nv, tri, nt, skin);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
 }  // extern "C"
 
 namespace {
@@ -1023,7 +1138,7 @@ void scene_layout(const Ctx* x, SnapBody* b) {
 
 extern "C" {
 
-PBF_API int pbf_save_snapshot(pbf_ctx* c, const char* path) {
+static int pbf_save_snapshot_impl(pbf_ctx* c, const char* path) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -1055,7 +1170,17 @@ PBF_API int pbf_save_snapshot(pbf_ctx* c, const char* path) {
   return PBF_OK;
 }
 
-PBF_API int pbf_load_snapshot(pbf_ctx* c, const char* path) {
+PBF_API int pbf_save_snapshot(pbf_ctx* c, const char* path) {
+  try {
+    return pbf_save_snapshot_impl(c, path);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_load_snapshot_impl(pbf_ctx* c, const char* path) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   if (!path) return fail(x, PBF_ERR_INVALID, "path is NULL");
@@ -1103,7 +1228,17 @@ PBF_API int pbf_load_snapshot(pbf_ctx* c, const char* path) {
   return PBF_OK;
 }
 
-PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
+PBF_API int pbf_load_snapshot(pbf_ctx* c, const char* path) {
+  try {
+    return pbf_load_snapshot_impl(c, path);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_estimate_densities_impl(pbf_ctx* c, float* rho) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -1119,6 +1254,16 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
   x->has_nbrs = false;
   x->field_ready = true;
   return PBF_OK;
+}
+
+PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
+  try {
+    return pbf_estimate_densities_impl(c, rho);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 // Density field of one scene at m query points (pts) or on a lattice.  The
@@ -1165,12 +1310,22 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
   return PBF_OK;
 }
 
-PBF_API int pbf_density_at(pbf_ctx* c, int32_t scene, const float* pts, int64_t m, float* out) {
+static int pbf_density_at_impl(pbf_ctx* c, int32_t scene, const float* pts, int64_t m, float* out) {
   return density_field(C(c), scene, pts, nullptr, m, out);
 }
 
-PBF_API int pbf_density_lattice(pbf_ctx* c, int32_t scene, const float* lo, const float* hi,
-                                float step, const int32_t* counts, float* out) {
+PBF_API int pbf_density_at(pbf_ctx* c, int32_t scene, const float* pts, int64_t m, float* out) {
+  try {
+    return pbf_density_at_impl(c, scene, pts, m, out);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+static int pbf_density_lattice_impl(pbf_ctx* c, int32_t scene, const float* lo, const float* hi,
+                                    float step, const int32_t* counts, float* out) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   if (!lo || !hi || !counts) return fail(x, PBF_ERR_INVALID, "NULL argument");
@@ -1190,6 +1345,17 @@ PBF_API int pbf_density_lattice(pbf_ctx* c, int32_t scene, const float* lo, cons
   return density_field(x, scene, nullptr, &L, m, out);
 }
 
+PBF_API int pbf_density_lattice(pbf_ctx* c, int32_t scene, const float* lo, const float* hi,
+                                float step, const int32_t* counts, float* out) {
+  try {
+    return pbf_density_lattice_impl(c, scene, lo, hi, step, counts, out);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
 PBF_API double pbf_simulate_time(const pbf_ctx* c) {
   const Ctx* x = C(c);
   return x ? x->sim_time : 0.0;
@@ -1206,7 +1372,7 @@ PBF_API const char* pbf_last_error(const pbf_ctx* c) {
   return g_err.c_str();
 }
 
-PBF_API int pbf_neighbors(pbf_ctx* c, uint32_t* counts, uint32_t* ids, int32_t stride) {
+static int pbf_neighbors_impl(pbf_ctx* c, uint32_t* counts, uint32_t* ids, int32_t stride) {
   Ctx* x = C(c);
   int rc = check_ready(x);
   if (rc != PBF_OK) return rc;
@@ -1230,6 +1396,16 @@ PBF_API int pbf_neighbors(pbf_ctx* c, uint32_t* counts, uint32_t* ids, int32_t s
   (void)hipFree(di);
   if (e != hipSuccess) return fail(x, PBF_ERR_HIP, std::string("neighbour export: ") + hipGetErrorString(e));
   return PBF_OK;
+}
+
+PBF_API int pbf_neighbors(pbf_ctx* c, uint32_t* counts, uint32_t* ids, int32_t stride) {
+  try {
+    return pbf_neighbors_impl(c, counts, ids, stride);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
 }
 
 PBF_API int pbf_neighbor_stats(pbf_ctx* c, uint64_t* total, uint32_t* max_count) {
