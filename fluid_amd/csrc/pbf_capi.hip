@@ -11,6 +11,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pbf_internal.h"
@@ -503,6 +504,8 @@ PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t
   }
 }
 
+static void field_stage_free(Ctx* x);
+
 PBF_API int pbf_destroy(pbf_ctx* c) {
   Ctx* x = C(c);
   if (!x) return PBF_OK;
@@ -513,6 +516,7 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   dfree(x->d.meshTri);
   dfree(x->d.meshNode);
   dfree(x->d.meshFlag);
+  field_stage_free(x);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return PBF_OK;
@@ -1268,6 +1272,62 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
 
 // Density field of one scene at m query points (pts) or on a lattice.  The
 // committed state is re-sorted by cell first, as pbf_estimate_densities does.
+// Pinned slabs of kFieldSlab floats through which density results return.
+constexpr size_t kFieldSlab = (size_t)16 << 20;
+
+static hipError_t field_stage_alloc(Ctx* x) {
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && e == hipSuccess; s++) {
+    e = hipHostMalloc((void**)&x->field_host[s], kFieldSlab * sizeof(float), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&x->field_ev[s], hipEventDisableTiming);
+  }
+  return e;
+}
+
+static void field_stage_free(Ctx* x) {
+  for (int s = 0; s < 2; s++) {
+    if (x->field_host[s]) (void)hipHostFree(x->field_host[s]);
+    if (x->field_ev[s]) (void)hipEventDestroy(x->field_ev[s]);
+    x->field_host[s] = nullptr;
+    x->field_ev[s] = nullptr;
+  }
+  dfree(x->field_dev);
+  x->field_cap = 0;
+}
+
+// Copies m results from field_dev to out: slab k+1 travels to its pinned
+// buffer while host threads copy slab k into out (a plain copy to pageable
+// memory is staged and copied by one thread, and touches every new page on
+// that thread).
+static hipError_t field_copy_back(Ctx* x, size_t m, float* out) {
+  const size_t nslab = (m + kFieldSlab - 1) / kFieldSlab;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nthr = std::max(1u, std::min(8u, hw ? hw : 1u));
+  for (size_t s = 0; s <= nslab; s++) {
+    if (s < nslab) {
+      const size_t off = s * kFieldSlab, cnt = std::min(kFieldSlab, m - off);
+      hipError_t e = hipMemcpyAsync(x->field_host[s & 1], x->field_dev + off, cnt * sizeof(float),
+                                    hipMemcpyDeviceToHost, x->stream);
+      if (e == hipSuccess) e = hipEventRecord(x->field_ev[s & 1], x->stream);
+      if (e != hipSuccess) return e;
+    }
+    if (s == 0) continue;
+    const size_t prev = s - 1, off = prev * kFieldSlab, cnt = std::min(kFieldSlab, m - off);
+    hipError_t e = hipEventSynchronize(x->field_ev[prev & 1]);
+    if (e != hipSuccess) return e;
+    const float* src = x->field_host[prev & 1];
+    const size_t per = (cnt + nthr - 1) / nthr;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nthr && t * per < cnt; t++)
+      th.emplace_back([=] {
+        memcpy(out + off + t * per, src + t * per, std::min(per, cnt - t * per) * sizeof(float));
+      });
+    memcpy(out + off, src, std::min(per, cnt) * sizeof(float));
+    for (auto& t : th) t.join();
+  }
+  return hipSuccess;
+}
+
 static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::DensityLattice* L,
                          int64_t m, float* out) {
   int rc = check_ready(x);
@@ -1281,20 +1341,25 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
       if (!is_finite_f(pts[i])) return fail(x, PBF_ERR_INVALID, "non-finite query point");
   if (m == 0) return PBF_OK;
   const SimConst k = pbf::consts_for(*x, x->p.dt);
-  float *dp = nullptr, *dout = nullptr;
-  hipError_t e = hipMalloc((void**)&dout, (size_t)m * sizeof(float));
+  float* dp = nullptr;
+  hipError_t e = hipSuccess;
+  if ((size_t)m > x->field_cap) {
+    dfree(x->field_dev);
+    x->field_cap = 0;
+    e = dalloc(x->field_dev, (size_t)m);
+    if (e == hipSuccess) x->field_cap = (size_t)m;
+  }
+  if (e == hipSuccess && !x->field_host[0]) e = field_stage_alloc(x);
   if (e == hipSuccess && !L) e = hipMalloc((void**)&dp, (size_t)m * 3 * sizeof(float));
   if (e == hipSuccess && !L)
     e = hipMemcpyAsync(dp, pts, (size_t)m * 3 * sizeof(float), hipMemcpyHostToDevice, x->stream);
   // Sorted once per committed state: later queries reuse the cell table.
   const bool sort = !x->field_ready;
   if (e == hipSuccess && sort) e = pbf::launch_sort_committed(*x, k);
-  if (e == hipSuccess) e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, dout);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(out, dout, (size_t)m * sizeof(float), hipMemcpyDeviceToHost, x->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+    e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, x->field_dev);
+  if (e == hipSuccess) e = field_copy_back(x, (size_t)m, out);
   (void)hipFree(dp);
-  (void)hipFree(dout);
   if (e != hipSuccess) {
     x->field_ready = false;
     return fail(x, PBF_ERR_HIP, std::string("density field: ") + hipGetErrorString(e));
