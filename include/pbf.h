@@ -174,7 +174,10 @@ PBF_API int pbf_load_snapshot(pbf_ctx *ctx, const char *path);
  * the particles of scene `scene` (0 unless batched) of poly6(x_j - p_q),
  * evaluated on the cell grid: particles of the 27 cells around p_q in
  * (cell, id) order.  pts: m*3 floats.  Like pbf_estimate_densities it
- * re-sorts the committed state (the positions do not change). */
+ * re-sorts the committed state (the positions do not change).  The first
+ * call allocates two 64 MB pinned host slabs and a device buffer of m
+ * floats (grown when a later call needs more); the context keeps them
+ * until pbf_destroy. */
 PBF_API int pbf_density_at(pbf_ctx *ctx, int32_t scene, const float *pts, int64_t m, float *out);
 
 /* The same field on the lattice of marching-cubes corners: point (ix, iy,
