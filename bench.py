@@ -19,8 +19,9 @@ HBM.  Prints ONE JSON line:
              mean launch time, against 157.3 TFLOP/s as 78.6e12 lane
              operations per second;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
-* density_field, snapshot, mesh, batched, adjoint  the paths beside the step
-             (marching-cubes density lattice, checkpoint files, the
+* readback, density_field, snapshot, mesh, batched, adjoint  the paths
+             beside the step (positions for the CPU consumers, the
+             marching-cubes density lattice, checkpoint files, the
              triangle-mesh collider, many small scenes, gradients);
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
@@ -365,6 +366,13 @@ def main() -> int:
     wall_profiled = time.perf_counter() - t0
     log(f"profiled {args.steps} steps: {wall_profiled:.3f} s")
     nbar, nmax = ctx.neighbor_stats()
+    ctx.read()  # first call allocates the pinned staging slabs
+    t0 = time.perf_counter()
+    ctx.read()
+    read_s = time.perf_counter() - t0
+    readback = {"metric": "pbf_read of x, v, rho (load order) into new host arrays",
+                "bytes": 28 * sc.n, "ms": read_s * 1e3, "GBs": 28 * sc.n / read_s / 1e9}
+    log(f"read back x, v, rho: {read_s * 1e3:.1f} ms")
     field = density_field_bench(ctx, sc)
     snap = snapshot_bench(ctx, sc)
     ctx.close()
@@ -449,6 +457,7 @@ def main() -> int:
         "roofline": roof,
         "ms_per_step_profiled": wall_profiled / args.steps * 1e3,
         "passes": passes,
+        "readback": readback,
         "density_field": field,
         "snapshot": snap,
     }

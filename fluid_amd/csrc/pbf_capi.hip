@@ -325,7 +325,75 @@ int check_ready(Ctx* x) {
   return PBF_OK;
 }
 
+// Pinned slabs of kStageSlab floats through which large results return.
+constexpr size_t kStageSlab = (size_t)16 << 20;
+
+hipError_t stage_alloc(Ctx* x) {
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && e == hipSuccess; s++) {
+    e = hipHostMalloc((void**)&x->stage_host[s], kStageSlab * sizeof(float), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&x->stage_ev[s], hipEventDisableTiming);
+  }
+  return e;
+}
+
+void stage_free(Ctx* x) {
+  for (int s = 0; s < 2; s++) {
+    if (x->stage_host[s]) (void)hipHostFree(x->stage_host[s]);
+    if (x->stage_ev[s]) (void)hipEventDestroy(x->stage_ev[s]);
+    x->stage_host[s] = nullptr;
+    x->stage_ev[s] = nullptr;
+  }
+}
+
+// Copies m floats from device memory to out: slab k+1 travels to its pinned
+// buffer while host threads copy slab k into out (a plain copy to pageable
+// memory is staged and copied by one thread, and takes every first-touch
+// page fault of a new array on that thread).
+hipError_t stage_copy_back(Ctx* x, const float* dsrc, size_t m, float* out) {
+  if (!x->stage_host[0]) {
+    const hipError_t e = stage_alloc(x);
+    if (e != hipSuccess) {
+      stage_free(x);
+      return e;
+    }
+  }
+  const size_t nslab = (m + kStageSlab - 1) / kStageSlab;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nthr = std::max(1u, std::min(8u, hw ? hw : 1u));
+  for (size_t s = 0; s <= nslab; s++) {
+    if (s < nslab) {
+      const size_t off = s * kStageSlab, cnt = std::min(kStageSlab, m - off);
+      hipError_t e = hipMemcpyAsync(x->stage_host[s & 1], dsrc + off, cnt * sizeof(float),
+                                    hipMemcpyDeviceToHost, x->stream);
+      if (e == hipSuccess) e = hipEventRecord(x->stage_ev[s & 1], x->stream);
+      if (e != hipSuccess) return e;
+    }
+    if (s == 0) continue;
+    const size_t prev = s - 1, off = prev * kStageSlab, cnt = std::min(kStageSlab, m - off);
+    hipError_t e = hipEventSynchronize(x->stage_ev[prev & 1]);
+    if (e != hipSuccess) return e;
+    const float* src = x->stage_host[prev & 1];
+    const size_t per = (cnt + nthr - 1) / nthr;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nthr && t * per < cnt; t++)
+      th.emplace_back([=] {
+        memcpy(out + off + t * per, src + t * per, std::min(per, cnt - t * per) * sizeof(float));
+      });
+    memcpy(out + off, src, std::min(per, cnt) * sizeof(float));
+    for (auto& t : th) t.join();
+  }
+  return hipSuccess;
+}
+
+// Device-to-host copy of `bytes` (a whole number of floats) on the context
+// stream, synchronous at return; copies of 4 MB or more go through the
+// pinned slabs.
 int copy_out(Ctx* x, const float* dsrc, void* host, size_t bytes) {
+  if (bytes >= ((size_t)4 << 20)) {
+    HIPCHK(x, stage_copy_back(x, dsrc, bytes / sizeof(float), static_cast<float*>(host)));
+    return PBF_OK;
+  }
   HIPCHK(x, hipMemcpyAsync(host, dsrc, bytes, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(x, hipStreamSynchronize(x->stream));
   return PBF_OK;
@@ -504,8 +572,6 @@ PBF_API int pbf_create_batch(const pbf_params* p, int32_t nscenes, const int64_t
   }
 }
 
-static void field_stage_free(Ctx* x);
-
 PBF_API int pbf_destroy(pbf_ctx* c) {
   Ctx* x = C(c);
   if (!x) return PBF_OK;
@@ -516,7 +582,8 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   dfree(x->d.meshTri);
   dfree(x->d.meshNode);
   dfree(x->d.meshFlag);
-  field_stage_free(x);
+  stage_free(x);
+  dfree(x->field_dev);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return PBF_OK;
@@ -1272,62 +1339,6 @@ PBF_API int pbf_estimate_densities(pbf_ctx* c, float* rho) {
 
 // Density field of one scene at m query points (pts) or on a lattice.  The
 // committed state is re-sorted by cell first, as pbf_estimate_densities does.
-// Pinned slabs of kFieldSlab floats through which density results return.
-constexpr size_t kFieldSlab = (size_t)16 << 20;
-
-static hipError_t field_stage_alloc(Ctx* x) {
-  hipError_t e = hipSuccess;
-  for (int s = 0; s < 2 && e == hipSuccess; s++) {
-    e = hipHostMalloc((void**)&x->field_host[s], kFieldSlab * sizeof(float), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&x->field_ev[s], hipEventDisableTiming);
-  }
-  return e;
-}
-
-static void field_stage_free(Ctx* x) {
-  for (int s = 0; s < 2; s++) {
-    if (x->field_host[s]) (void)hipHostFree(x->field_host[s]);
-    if (x->field_ev[s]) (void)hipEventDestroy(x->field_ev[s]);
-    x->field_host[s] = nullptr;
-    x->field_ev[s] = nullptr;
-  }
-  dfree(x->field_dev);
-  x->field_cap = 0;
-}
-
-// Copies m results from field_dev to out: slab k+1 travels to its pinned
-// buffer while host threads copy slab k into out (a plain copy to pageable
-// memory is staged and copied by one thread, and touches every new page on
-// that thread).
-static hipError_t field_copy_back(Ctx* x, size_t m, float* out) {
-  const size_t nslab = (m + kFieldSlab - 1) / kFieldSlab;
-  const unsigned hw = std::thread::hardware_concurrency();
-  const unsigned nthr = std::max(1u, std::min(8u, hw ? hw : 1u));
-  for (size_t s = 0; s <= nslab; s++) {
-    if (s < nslab) {
-      const size_t off = s * kFieldSlab, cnt = std::min(kFieldSlab, m - off);
-      hipError_t e = hipMemcpyAsync(x->field_host[s & 1], x->field_dev + off, cnt * sizeof(float),
-                                    hipMemcpyDeviceToHost, x->stream);
-      if (e == hipSuccess) e = hipEventRecord(x->field_ev[s & 1], x->stream);
-      if (e != hipSuccess) return e;
-    }
-    if (s == 0) continue;
-    const size_t prev = s - 1, off = prev * kFieldSlab, cnt = std::min(kFieldSlab, m - off);
-    hipError_t e = hipEventSynchronize(x->field_ev[prev & 1]);
-    if (e != hipSuccess) return e;
-    const float* src = x->field_host[prev & 1];
-    const size_t per = (cnt + nthr - 1) / nthr;
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nthr && t * per < cnt; t++)
-      th.emplace_back([=] {
-        memcpy(out + off + t * per, src + t * per, std::min(per, cnt - t * per) * sizeof(float));
-      });
-    memcpy(out + off, src, std::min(per, cnt) * sizeof(float));
-    for (auto& t : th) t.join();
-  }
-  return hipSuccess;
-}
-
 static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::DensityLattice* L,
                          int64_t m, float* out) {
   int rc = check_ready(x);
@@ -1349,7 +1360,6 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
     e = dalloc(x->field_dev, (size_t)m);
     if (e == hipSuccess) x->field_cap = (size_t)m;
   }
-  if (e == hipSuccess && !x->field_host[0]) e = field_stage_alloc(x);
   if (e == hipSuccess && !L) e = hipMalloc((void**)&dp, (size_t)m * 3 * sizeof(float));
   if (e == hipSuccess && !L)
     e = hipMemcpyAsync(dp, pts, (size_t)m * 3 * sizeof(float), hipMemcpyHostToDevice, x->stream);
@@ -1358,7 +1368,7 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
   if (e == hipSuccess && sort) e = pbf::launch_sort_committed(*x, k);
   if (e == hipSuccess)
     e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, x->field_dev);
-  if (e == hipSuccess) e = field_copy_back(x, (size_t)m, out);
+  if (e == hipSuccess) e = stage_copy_back(x, x->field_dev, (size_t)m, out);
   (void)hipFree(dp);
   if (e != hipSuccess) {
     x->field_ready = false;

@@ -85,12 +85,12 @@ struct Ctx {
   int fast_div = 0;                  // short division by h^9 verified exact
   int fast_rcp = 0;                  // short reciprocal verified exact
   int fast_sqrt = 0;                 // short square root verified exact
-  // Density-field output: a device buffer kept between calls and two pinned
-  // host slabs through which results return (allocated on first use).
+  // Density-field output on the device, kept between calls, and two pinned
+  // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;
   size_t field_cap = 0;
-  float* field_host[2] = {nullptr, nullptr};
-  hipEvent_t field_ev[2] = {nullptr, nullptr};
+  float* stage_host[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)
   hipGraphExec_t graph_exec = nullptr;
   float graph_dt = 0.0f;
