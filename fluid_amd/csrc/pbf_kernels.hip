@@ -304,20 +304,35 @@ __device__ __forceinline__ void scatter_one(uint32_t i, const Dev& d) {
 // Rank of a particle inside its cell = number of cell members with a smaller
 // id; the gather target is cellStart + rank, which makes the order (cell, id)
 // independent of atomic arrival order.
+// Everything the particle moves is loaded before the count, and the cell's
+// ids are read four at a time with independent loads, so the loads of one
+// particle overlap instead of waiting on each other.
 __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d,
                                          const float4* __restrict__ pred) {
+  const float4 x = d.posA[i];
+  const float4 v = d.velA[i];
+  const float4 q = pred[i];
   const uint32_t c = d.cell[i];
   const uint32_t s = d.cellStart[c];
   const uint32_t e = min(d.cellStart[c + 1], d.n);
-  const float4 x = d.posA[i];
   const uint32_t id = id_of(x);
+  const uint32_t* __restrict__ ids = d.tmpId;
   uint32_t rank = 0;
-  for (uint32_t t = s; t < e; t++) rank += (d.tmpId[t] < id) ? 1u : 0u;
+  const uint32_t last = e > s ? e - 1 : s;
+  for (uint32_t t = s; t < e; t += 4) {
+    // Slots past the end re-read the last member and do not count.
+    const uint32_t a0 = ids[t];
+    const uint32_t a1 = ids[min(t + 1, last)];
+    const uint32_t a2 = ids[min(t + 2, last)];
+    const uint32_t a3 = ids[min(t + 3, last)];
+    rank += (uint32_t)(a0 < id) + (uint32_t)(t + 1 < e && a1 < id) +
+            (uint32_t)(t + 2 < e && a2 < id) + (uint32_t)(t + 3 < e && a3 < id);
+  }
   const uint32_t dst = s + rank;
   if (dst >= d.n) return;
   d.posB[dst] = x;
-  d.velB[dst] = d.velA[i];
-  d.predQ[dst] = pred[i];
+  d.velB[dst] = v;
+  d.predQ[dst] = q;
   d.cellSorted[dst] = c;
 }
 
