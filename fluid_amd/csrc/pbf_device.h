@@ -427,8 +427,7 @@ __device__ __forceinline__ size_t nbr_slot(uint32_t i, int k, int K4) {
 
 // One 4-entry list chunk.  Lists are read once per pass, so the load is
 // non-temporal: it does not displace gathered neighbour state in the L1
-// (λ -4%, Δp -4%, build + first λ -3%; the grouped visit of XSPH gained
-// nothing and keeps plain loads).
+// (λ -4%, Δp -4%, build + first λ -3%).
 #ifndef PBF_NT_INDEX
 #define PBF_NT_INDEX 1
 #endif

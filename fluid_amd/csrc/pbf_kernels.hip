@@ -35,11 +35,7 @@ constexpr int kBlock = 256;
 #ifndef PBF_UNROLL
 #define PBF_UNROLL 4
 #endif
-#ifndef PBF_UNROLL_XSPH
-#define PBF_UNROLL_XSPH 8
-#endif
-constexpr int kUnroll = PBF_UNROLL;           // lambda, delta-p, vorticity
-constexpr int kUnrollXsph = PBF_UNROLL_XSPH;  // XSPH gathers a 24-byte record per neighbour
+constexpr int kUnroll = PBF_UNROLL;
 
 // Device pointers and sizes of one context, passed by value to every kernel.
 struct Dev {
@@ -544,7 +540,7 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
   struct XV {
     float3 x, v;
   };
-  for_each_neighbor<kUnrollXsph>(
+  visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
       [&](uint32_t, const XV& q) {
         const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
