@@ -100,10 +100,13 @@ def durations(path: str):
     return {key: sum(v) / len(v) for key, v in acc.items()}
 
 
-def calibration(table, counter: str) -> float:
-    """HBM bytes per counted KiB, from the largest copyBuffer dispatch."""
-    best = max((v[counter] for (k, _), v in table.items() if k.startswith("__amd_rocclr_copyBuffer")),
-               default=0.0)
+def calibration(path: str, counter: str) -> float:
+    """HBM bytes per counted KiB, from the largest single copyBuffer dispatch
+    (the 256 MiB calibration copy; the bench's later copies are smaller)."""
+    best = max((float(r["Counter_Value"])
+                for r in read_csv(os.path.join(path, "run_counter_collection.csv"))
+                if short(r["Kernel_Name"]).startswith("__amd_rocclr_copyBuffer")
+                and r["Counter_Name"] == counter), default=0.0)
     if best <= 0:
         raise SystemExit(f"no calibration copy found for {counter}: run bench.py with --calibrate")
     return COPY_BYTES / best
@@ -125,7 +128,7 @@ def counters(args) -> None:
     ta = per_kernel(args.ta) if args.ta else {}
     ncu, nxcd = gpu_shape(args.ta) if args.ta else (0, 0)
     dur = durations(args.sq)
-    ff, wf = calibration(fetch, "FETCH_SIZE"), calibration(write, "WRITE_SIZE")
+    ff, wf = calibration(args.fetch, "FETCH_SIZE"), calibration(args.write, "WRITE_SIZE")
     out = {"workload": args.workload, "fetch_factor": ff, "write_factor": wf, "passes": {},
            "kernels": {}}
     lines = [f"# rocprofv3 --pmc summary: {args.workload}", "",
