@@ -1084,7 +1084,8 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
   // boxes are grown by 3 pad here (a hit point lies within pad/2 of its
   // triangle's box, hit_in_box), far more than the float cell indexing on
   // the device can err; if marking would visit more than 64 cells per grid
-  // cell, every cell is flagged (the walk always runs).
+  // cell, every cell is flagged (the walk always runs).  The flags are then
+  // dilated by one cell.
   // PBF_MESH_CELLS=0 leaves the cells out (every segment walks the BVH).
   const pbf::SceneGrid& g = x->grids[0];
   const char* cells_env = getenv("PBF_MESH_CELLS");
@@ -1119,6 +1120,21 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
             flag[((size_t)z * g.dims[1] + y) * g.dims[0] + xx] = 1;
     }
     if (all) std::fill(flag.begin(), flag.end(), (uint8_t)1);
+    // Dilate by one cell on every axis (three passes of a 3-wide OR): the
+    // device then reads only the start cell of a displacement.
+    const size_t nx = (size_t)g.dims[0], ny = (size_t)g.dims[1], nz = (size_t)g.dims[2];
+    const size_t stride[3] = {1, nx, nx * ny}, dim[3] = {nx, ny, nz};
+    std::vector<uint8_t> tmp(ncell);
+    for (int a = 0; a < 3 && !all; a++) {
+      for (size_t c = 0; c < ncell; c++) {
+        const size_t k = (c / stride[a]) % dim[a];
+        uint8_t v = flag[c];
+        if (k > 0) v |= flag[c - stride[a]];
+        if (k + 1 < dim[a]) v |= flag[c + stride[a]];
+        tmp[c] = v;
+      }
+      flag.swap(tmp);
+    }
   }
   hipError_t e = dalloc(x->d.meshTri, tv.size());
   if (e == hipSuccess) e = dalloc(x->d.meshNode, mb.nodes.size());

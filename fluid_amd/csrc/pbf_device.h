@@ -90,8 +90,9 @@ struct MeshView {
   const float4* node = nullptr;
   uint32_t ntri = 0;  // 0: the analytic box walls
   float skin = 0.0f;  // clearance a hit leaves along the face normal
-  // One byte per cell of the scene grid: nonzero where a BVH leaf box, grown
-  // by 2 margin, overlaps the cell (NULL: always walk the BVH).
+  // One byte per cell of the scene grid: nonzero where a triangle's box,
+  // grown by 3 margins, overlaps the cell or one of its 26 neighbours
+  // (NULL: always walk the BVH).
   const uint8_t* flag = nullptr;
   float margin = 0.0f;
 };
@@ -201,17 +202,19 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
 }
 
 // Whether the segment o + t d, 0 <= t <= l, can hit the mesh.  A hit point
-// lies within margin/2 of its triangle's box (hit_in_box); the flagged
-// cells cover every triangle box grown by 3 margins, so a segment whose box,
-// grown by one margin, touches no flagged cell holds no such point (the
-// float cell indexing errs far less than a margin).  Segments that span
-// more than two cells on an axis, and non-finite ones, walk the BVH.
+// lies within margin/2 of its triangle's box (hit_in_box); the cells near a
+// triangle cover every triangle box grown by 3 margins, so a segment whose
+// box, grown by one margin, touches none of them holds no such point (the
+// float cell indexing errs far less than a margin).  The box spans at most
+// two cells per axis, one of them the start point's, so one byte, the start
+// cell's (flags dilated by one cell), decides.  Longer segments, and
+// non-finite ones, walk the BVH.
 __device__ __forceinline__ bool mesh_may_hit(const MeshView& m, const SceneGrid& g, float3 o,
                                              float3 d, float l) {
   if (!m.flag) return true;
   const float s[3] = {o.x, o.y, o.z};
   const float e[3] = {o.x + d.x * l, o.y + d.y * l, o.z + d.z * l};
-  int c0[3], c1[3];
+  int cs[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     const float lo = fminf(s[a], e[a]) - m.margin, hi = fmaxf(s[a], e[a]) + m.margin;
@@ -219,16 +222,14 @@ __device__ __forceinline__ bool mesh_may_hit(const MeshView& m, const SceneGrid&
         !(fabsf(e[a]) < 3.0e38f))
       return true;
     const float f0 = floorf((lo - g.lo[a]) * g.inv_cell), f1 = floorf((hi - g.lo[a]) * g.inv_cell);
-    c0[a] = f0 < 0.0f ? 0 : (f0 < (float)g.dims[a] ? (int)f0 : g.dims[a] - 1);
-    c1[a] = f1 < 0.0f ? 0 : (f1 < (float)g.dims[a] ? (int)f1 : g.dims[a] - 1);
-    if (c1[a] - c0[a] > 1) return true;
+    const float fs = floorf((s[a] - g.lo[a]) * g.inv_cell);
+    const int c0 = f0 < 0.0f ? 0 : (f0 < (float)g.dims[a] ? (int)f0 : g.dims[a] - 1);
+    const int c1 = f1 < 0.0f ? 0 : (f1 < (float)g.dims[a] ? (int)f1 : g.dims[a] - 1);
+    cs[a] = fs < 0.0f ? 0 : (fs < (float)g.dims[a] ? (int)fs : g.dims[a] - 1);
+    if (c1 - c0 > 1) return true;
   }
-  for (int z = c0[2]; z <= c1[2]; z++)
-    for (int y = c0[1]; y <= c1[1]; y++)
-      for (int x = c0[0]; x <= c1[0]; x++)
-        if (m.flag[((size_t)z * (size_t)g.dims[1] + (size_t)y) * (size_t)g.dims[0] + (size_t)x])
-          return true;
-  return false;
+  return m.flag[((size_t)cs[2] * (size_t)g.dims[1] + (size_t)cs[1]) * (size_t)g.dims[0] +
+                (size_t)cs[0]] != 0;
 }
 
 // Box collision along dir for a displacement D of length l (= total).
