@@ -957,9 +957,12 @@ PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
 namespace {
 // ---- mesh collider: BVH build on the host --------------------------------
 // Median split on the longest axis of the centroid bounds (ties by index, so
-// the tree is deterministic), leaves of at most 4 triangles, depth at most
-// 40 (the device traversal stack holds 64 entries).  Node boxes are padded
-// outwards by 1e-5 of the scene scale so the float slab test is
+// the tree is deterministic), leaves of at most 4 triangles.  Nodes are
+// stored in depth-first preorder, so an inner node's left child follows it,
+// and each node records its escape: the first node after its subtree.  The
+// device walks the tree without a stack (a missed box or a finished leaf
+// continues at the escape, a hit inner box at the next node).  Node boxes
+// are padded outwards by 1e-5 of the scene scale so the float slab test is
 // conservative.
 struct MeshBuild {
   const float* pos;
@@ -970,6 +973,7 @@ struct MeshBuild {
   float pad = 0.0f;
   int depth = 0;
 
+  // Returns the node's index; the subtree occupies [index, nodes.size()/2).
   uint32_t build(uint32_t b, uint32_t e, int level) {
     depth = std::max(depth, level);
     const uint32_t ni = (uint32_t)(nodes.size() / 2);
@@ -985,10 +989,9 @@ struct MeshBuild {
         ch[a] = std::max(ch[a], cen[3 * k + a]);
       }
     }
-    uint32_t ua, ub;
-    if (e - b <= 4 || level >= 40) {
-      ua = b;
-      ub = 0x80000000u | (e - b);
+    uint32_t ub = 0;  // inner node: 0; leaf: 2^31 | count << 27 | first triangle
+    if (e - b <= 4) {
+      ub = 0x80000000u | ((e - b) << 27) | b;
     } else {
       int ax = 0;
       for (int a = 1; a < 3; a++)
@@ -999,9 +1002,10 @@ struct MeshBuild {
                          const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
                          return cx < cy || (cx == cy && x < y);
                        });
-      ua = build(b, mid, level + 1);
-      ub = build(mid, e, level + 1);
+      build(b, mid, level + 1);  // node ni + 1
+      build(mid, e, level + 1);
     }
+    const uint32_t ua = (uint32_t)(nodes.size() / 2);  // escape
     float ufa, ufb;
     memcpy(&ufa, &ua, 4);
     memcpy(&ufb, &ub, 4);
@@ -1062,6 +1066,18 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
     scale = std::max(scale, std::max(fabsf(x->grids[0].lo[a]), fabsf(x->grids[0].hi[a])));
   mb.pad = 1e-5f * std::max(scale, 1e-6f);
   mb.build(0, (uint32_t)n, 0);
+  // The device walk moves to a larger index at every node, so it ends after
+  // at most nnode nodes; check the links that guarantee it.
+  const uint32_t nnode = (uint32_t)(mb.nodes.size() / 2);
+  for (uint32_t i = 0; i < nnode; i++) {
+    uint32_t esc, b;
+    memcpy(&esc, &mb.nodes[2 * i].w, 4);
+    memcpy(&b, &mb.nodes[2 * i + 1].w, 4);
+    const bool leaf = (b & 0x80000000u) != 0;
+    if (!(esc > i && esc <= nnode) || (!leaf && i + 1 >= nnode) ||
+        (leaf && (((b >> 27) & 7u) == 0 || (b & 0x7ffffffu) + ((b >> 27) & 7u) > n)))
+      return fail(x, PBF_ERR_INVALID, "mesh BVH: inconsistent node links");
+  }
   // Triangle records in leaf order: p1 (original index bits), e1, e2, n1..n3.
   std::vector<float4> tv(6 * n);
   for (size_t i = 0; i < n; i++) {
@@ -1153,6 +1169,7 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
     return fail(x, PBF_ERR_NOMEM, std::string("mesh upload: ") + hipGetErrorString(e));
   }
   x->mesh_ntri = (uint32_t)nt;
+  x->mesh_nnode = nnode;
   x->mesh_skin = skin;
   x->mesh_margin = mb.pad;
   return PBF_OK;

@@ -81,14 +81,17 @@ __device__ __forceinline__ float comp(const float3& v, int a) {
 // with src/bvh.cpp:142-192 and src/static_scene/triangle.cpp:21-80).
 // Triangle k: tri[6k] = (p1, original index bits), tri[6k+1] = e1 = p2 - p1,
 // tri[6k+2] = e2 = p3 - p1, tri[6k+3..5] = vertex normals n1, n2, n3.
-// BVH node: node[2i] = (lo, a), node[2i+1] = (hi, b) with a, b as integer
-// bits; b >= 2^31 marks a leaf of triangles [a, a + (b & 0x7fffffff)),
-// otherwise a and b are the children.  Node boxes are padded outwards, so
-// the box test never drops a triangle that the exact test would accept.
+// BVH node, in depth-first preorder: node[2i] = (lo, escape), node[2i+1] =
+// (hi, b) with integer bits; escape is the first node after i's subtree;
+// b >= 2^31 marks a leaf of the (b >> 27) & 7 triangles from b & (2^27 - 1),
+// otherwise (b = 0) the left child is node i + 1.  Node boxes are padded
+// outwards, so the box test never drops a triangle that the exact test
+// would accept.
 struct MeshView {
   const float4* tri = nullptr;
   const float4* node = nullptr;
   uint32_t ntri = 0;  // 0: the analytic box walls
+  uint32_t nnode = 0;
   float skin = 0.0f;  // clearance a hit leaves along the face normal
   // One byte per cell of the scene grid: nonzero where a triangle's box,
   // grown by 3 margins, overlaps the cell or one of its 26 neighbours
@@ -135,12 +138,13 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
                                    float3* n_out, float3* nf_out) {
   float bt = tmax;
   int best = -1;
-  uint32_t stack[64];
-  int sp = 0;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    const uint32_t ni = stack[--sp];
+  // Stackless walk: a missed box or a finished leaf continues at the escape
+  // node, a hit inner box at its left child (the next node).  Every move
+  // goes to a larger index, so the walk ends within nnode nodes.
+  uint32_t ni = 0;
+  while (ni < m.nnode) {
     const float4 lo = m.node[2 * ni], hi = m.node[2 * ni + 1];
+    const uint32_t esc = __float_as_uint(lo.w), b = __float_as_uint(hi.w);
     // Slab test against [0, bt]; an axis the segment runs parallel to only
     // needs the origin inside the slab.
     float t0 = 0.0f, t1 = bt;
@@ -165,10 +169,12 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
     }
     // Rounding of the slab bounds is covered by the padded boxes; accept
     // a box whose entry is within a relative 1e-6 of the current bound.
-    if (miss || t0 > t1 * 1.000001f + 1e-30f) continue;
-    const uint32_t a = __float_as_uint(lo.w), b = __float_as_uint(hi.w);
+    if (miss || t0 > t1 * 1.000001f + 1e-30f) {
+      ni = esc;
+      continue;
+    }
     if (b & 0x80000000u) {
-      const uint32_t end = a + (b & 0x7fffffffu);
+      const uint32_t a = b & 0x7ffffffu, end = a + ((b >> 27) & 7u);
       for (uint32_t k = a; k < end; k++) {
         const float4 P = m.tri[6 * (size_t)k];
         const float3 e1 = f3(m.tri[6 * (size_t)k + 1]), e2 = f3(m.tri[6 * (size_t)k + 2]);
@@ -192,9 +198,9 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
                              (w * n1.z + u * n2.z) + v * n3.z);
         *nf_out = cross3(e1, e2);
       }
-    } else if (sp + 2 <= 64) {
-      stack[sp++] = b;
-      stack[sp++] = a;
+      ni = esc;
+    } else {
+      ni = ni + 1;
     }
   }
   *t_out = bt;

@@ -75,6 +75,7 @@ struct Ctx {
   bool has_nbrs = false;
   bool field_ready = false;          // committed state sorted by cell, cell table current
   uint32_t mesh_ntri = 0;            // triangles of the mesh collider (0: box walls)
+  uint32_t mesh_nnode = 0;           // BVH nodes
   float mesh_skin = 0.0f;            // clearance a mesh hit leaves along the face normal
   float mesh_margin = 0.0f;          // MeshView::margin
   const float4* dbg_pred = nullptr;  // x* buffer current at the last debug stop

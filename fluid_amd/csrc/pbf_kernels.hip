@@ -64,6 +64,7 @@ Dev dev_of(const Ctx& x) {
   d.mesh.tri = s.meshTri;
   d.mesh.node = s.meshNode;
   d.mesh.ntri = x.mesh_ntri;
+  d.mesh.nnode = x.mesh_nnode;
   d.mesh.skin = x.mesh_skin;
   d.mesh.flag = s.meshFlag;
   d.mesh.margin = x.mesh_margin;
