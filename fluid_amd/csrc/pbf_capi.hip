@@ -971,11 +971,9 @@ struct MeshBuild {
   std::vector<float> cen, lo, hi;     // per triangle: centroid, box
   std::vector<float4> nodes;          // 2 per node
   float pad = 0.0f;
-  int depth = 0;
 
   // Returns the node's index; the subtree occupies [index, nodes.size()/2).
-  uint32_t build(uint32_t b, uint32_t e, int level) {
-    depth = std::max(depth, level);
+  uint32_t build(uint32_t b, uint32_t e) {
     const uint32_t ni = (uint32_t)(nodes.size() / 2);
     nodes.resize(nodes.size() + 2);
     float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1002,8 +1000,8 @@ struct MeshBuild {
                          const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
                          return cx < cy || (cx == cy && x < y);
                        });
-      build(b, mid, level + 1);  // node ni + 1
-      build(mid, e, level + 1);
+      build(b, mid);  // node ni + 1
+      build(mid, e);
     }
     const uint32_t ua = (uint32_t)(nodes.size() / 2);  // escape
     float ufa, ufb;
@@ -1065,7 +1063,7 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
   for (int a = 0; a < 3; a++)
     scale = std::max(scale, std::max(fabsf(x->grids[0].lo[a]), fabsf(x->grids[0].hi[a])));
   mb.pad = 1e-5f * std::max(scale, 1e-6f);
-  mb.build(0, (uint32_t)n, 0);
+  mb.build(0, (uint32_t)n);
   // The device walk moves to a larger index at every node, so it ends after
   // at most nnode nodes; check the links that guarantee it.
   const uint32_t nnode = (uint32_t)(mb.nodes.size() / 2);
