@@ -144,7 +144,9 @@ PBF_API int pbf_sync(pbf_ctx *ctx);
  * may be NULL.  rho is the SPH density of the last step's XSPH pass (the
  * value the reference keeps in Particle::density, src/particles.cpp:222-231).
  * Replaces the reads of ps[i]->getPosition() / velocity / density by the
- * CPU consumers (src/particles.cpp:305, :449, src/particles.h:36-52). */
+ * CPU consumers (src/particles.cpp:305, :449, src/particles.h:36-52).
+ * Arrays of 4 MB or more return through two 64 MB pinned host slabs,
+ * allocated on first use and kept by the context until pbf_destroy. */
 PBF_API int pbf_read(pbf_ctx *ctx, float *xyz, float *v, float *rho);
 
 /* Load-time density including self, the reference's
@@ -175,9 +177,9 @@ PBF_API int pbf_load_snapshot(pbf_ctx *ctx, const char *path);
  * evaluated on the cell grid: particles of the 27 cells around p_q in
  * (cell, id) order.  pts: m*3 floats.  Like pbf_estimate_densities it
  * re-sorts the committed state (the positions do not change).  The first
- * call allocates two 64 MB pinned host slabs and a device buffer of m
- * floats (grown when a later call needs more); the context keeps them
- * until pbf_destroy. */
+ * call allocates a device buffer of m floats (grown when a later call needs
+ * more); results return through the pinned slabs of pbf_read.  The context
+ * keeps both until pbf_destroy. */
 PBF_API int pbf_density_at(pbf_ctx *ctx, int32_t scene, const float *pts, int64_t m, float *out);
 
 /* The same field on the lattice of marching-cubes corners: point (ix, iy,
