@@ -76,6 +76,32 @@ def test_density_lattice_matches_points(orc, stepped):
     assert bits_equal(lat.reshape(-1), orc.density_at(p, x, pts, orc.NBR_GRID))
 
 
+def test_density_lattice_across_copy_slabs_and_buffer_reuse(stepped):
+    """A lattice of 17.6M points returns through two pinned 64 MB slabs; a
+    smaller lattice before and after it (the device buffer grows, then is
+    reused) gives the same bits both times, and sampled corners of the large
+    one equal explicit points."""
+    p, x, v = stepped
+    lo = np.array([0.0, 0.0, 0.0], np.float32)
+    hi = np.array(p.box_hi, np.float32)
+    counts = [260, 260, 260]
+    step = np.float32(float(hi.min()) / 300.0)
+    with Particles(p) as c:
+        c.load(x, v)
+        small = c.density_lattice(lo, hi, float(p.h * 0.5), [8, 9, 10])
+        big = c.density_lattice(lo, hi, float(step), counts)
+        again = c.density_lattice(lo, hi, float(p.h * 0.5), [8, 9, 10])
+        rng = np.random.default_rng(3)
+        idx = rng.integers(0, [counts[2], counts[1], counts[0]], size=(2000, 3))
+        pts = np.stack([np.minimum(hi[a], lo[a] + idx[:, 2 - a].astype(np.float32) * step)
+                        for a in range(3)], -1).astype(np.float32)
+        at = c.density_at(pts)
+    assert bits_equal(small, again)
+    assert big.shape == (260, 260, 260)
+    assert bits_equal(big[idx[:, 0], idx[:, 1], idx[:, 2]], at)
+    assert np.count_nonzero(big) > 1000
+
+
 def test_density_at_particles_is_load_time_density(stepped):
     p, x, v = stepped
     with Particles(p) as c:

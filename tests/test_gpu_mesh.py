@@ -95,12 +95,16 @@ def test_box_mesh_matches_box_walls_and_removal_restores_them():
     assert bits_equal(cx, ax) and bits_equal(cv, av)
 
 
-def test_mesh_cell_pruning_changes_no_bit(monkeypatch):
-    """Segments whose box touches no grid cell near a BVH leaf skip the walk;
+@pytest.mark.parametrize("center", [(0.5, 0.3, 0.5), (0.95, 0.3, 0.5)])
+def test_mesh_cell_pruning_changes_no_bit(monkeypatch, center):
+    """Segments whose box touches no grid cell near a triangle skip the walk;
     with the cells switched off (PBF_MESH_CELLS=0) every segment walks it.
     Random velocities up to 3 m/s (0.3 cells per step) and a few particles
-    at 30 m/s (segments over two cells long) hit the sphere and the walls."""
+    at 30 m/s (segments over two cells long) hit the sphere and the walls;
+    the second sphere reaches out of the box (its cells clamp to the grid)."""
     p, x, _, m = drop_scene()
+    if center != (0.5, 0.3, 0.5):
+        m = mesh.merge(mesh.box((0, 0, 0), (1, 1, 1)), mesh.icosphere(center, 0.2, 2)) + (m[3],)
     rng = np.random.default_rng(11)
     v = rng.uniform(-3.0, 3.0, size=x.shape).astype(np.float32)
     v[::37] *= 10.0
@@ -121,7 +125,7 @@ def test_mesh_cell_pruning_changes_no_bit(monkeypatch):
     walked, pruned = run(False), run(True)
     for a, b in zip(walked, pruned):
         assert all(bits_equal(u, w) for u, w in zip(a, b))
-    r = np.linalg.norm(pruned[-1][0] - np.array([0.5, 0.3, 0.5]), axis=1)
+    r = np.linalg.norm(pruned[-1][0] - np.array(center), axis=1)
     assert r.min() > 0.18
 
 
