@@ -1188,7 +1188,10 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
 
 namespace {
 // Snapshot file layout (little-endian), see pbf_save_snapshot in pbf.h.
-constexpr char kSnapMagic[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '1'};
+// Version 2 checksums the payload with Fnv4 (written); version 1 files,
+// checksummed with byte-serial FNV-1a-64, still load.
+constexpr char kSnapMagic[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '2'};
+constexpr char kSnapMagic1[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '1'};
 struct SnapHeader {
   char magic[8];
   uint32_t header_bytes;  // sizeof(SnapHeader) = 64
@@ -1197,16 +1200,66 @@ struct SnapHeader {
   int32_t nscenes;
   int32_t reserved0;
   double sim_time;
-  uint64_t checksum;      // FNV-1a 64 of every byte after the header
+  uint64_t checksum;      // Fnv4 (version 2) of every byte after the header
   uint64_t reserved1[2];
 };
 static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
 
+// FNV-1a-64 prime and offset basis; version 1 files used the basis
+// 1469598103934665603 instead.
+constexpr uint64_t kFnvPrime = 1099511628211ull, kFnvBasis = 14695981039346656037ull;
+constexpr uint64_t kFnvBasisV1 = 1469598103934665603ull;
+
+// Byte-serial FNV-1a-64 from basis h0.
 struct Fnv {
-  uint64_t h = 1469598103934665603ull;
+  uint64_t h;
+  explicit Fnv(uint64_t h0 = kFnvBasis) : h(h0) {}
   void add(const void* p, size_t n) {
     const unsigned char* b = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * kFnvPrime;
+  }
+  uint64_t value() const { return h; }
+};
+
+// Fnv4: the FNV-1a step on 64-bit little-endian words in four lanes, so the
+// four multiply chains run side by side.  Over the payload as one byte
+// stream: 32-byte block k gives word l (bytes 32k + 8l ..) to lane l; the
+// bytes after the last whole block go byte by byte into lane 0; the value is
+// byte-serial FNV-1a-64 of the four lane values (32 bytes, lane 0 first).
+// Every lane step is a bijection, so any change confined to one lane's words
+// changes the value.
+struct Fnv4 {
+  uint64_t h[4] = {kFnvBasis, kFnvBasis, kFnvBasis, kFnvBasis};
+  unsigned char buf[32];
+  size_t nbuf = 0;
+  void block(const unsigned char* b) {
+    uint64_t w[4];
+    memcpy(w, b, 32);
+    for (int l = 0; l < 4; l++) h[l] = (h[l] ^ w[l]) * kFnvPrime;
+  }
+  void add(const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    while (n > 0 && nbuf > 0) {
+      buf[nbuf++] = *b++;
+      n--;
+      if (nbuf == 32) {
+        block(buf);
+        nbuf = 0;
+      }
+    }
+    for (; n >= 32; n -= 32, b += 32) block(b);
+    while (n > 0) {
+      buf[nbuf++] = *b++;
+      n--;
+    }
+  }
+  uint64_t value() const {
+    uint64_t t = h[0];
+    for (size_t i = 0; i < nbuf; i++) t = (t ^ buf[i]) * kFnvPrime;
+    const uint64_t lanes[4] = {t, h[1], h[2], h[3]};
+    Fnv f;
+    f.add(lanes, sizeof lanes);
+    return f.value();
   }
 };
 
@@ -1260,9 +1313,9 @@ static int pbf_save_snapshot_impl(pbf_ctx* c, const char* path) {
   hd.n = x->n;
   hd.nscenes = x->nscenes;
   hd.sim_time = x->sim_time;
-  Fnv f;
+  Fnv4 f;
   b.each([&](const void* p, size_t k) { f.add(p, k); });
-  hd.checksum = f.h;
+  hd.checksum = f.value();
   FILE* fp = fopen(path, "wb");
   if (!fp) return fail(x, PBF_ERR_INVALID, std::string("cannot open ") + path + " for writing");
   bool ok = fwrite(&hd, sizeof hd, 1, fp) == 1;
@@ -1291,7 +1344,9 @@ static int pbf_load_snapshot_impl(pbf_ctx* c, const char* path) {
   SnapHeader hd{};
   const bool head_ok = fread(&hd, sizeof hd, 1, fp) == 1;
   std::string why;
-  if (!head_ok || memcmp(hd.magic, kSnapMagic, 8) != 0 || hd.header_bytes != sizeof(SnapHeader))
+  const bool v1 = head_ok && memcmp(hd.magic, kSnapMagic1, 8) == 0;
+  if (!head_ok || (memcmp(hd.magic, kSnapMagic, 8) != 0 && !v1) ||
+      hd.header_bytes != sizeof(SnapHeader))
     why = "not a snapshot file";
   else if (hd.params_bytes != sizeof(pbf_params))
     why = "snapshot written with a different pbf_params layout";
@@ -1308,10 +1363,18 @@ static int pbf_load_snapshot_impl(pbf_ctx* c, const char* path) {
     b.rho.resize(n);
     bool ok = true;
     b.each([&](void* p, size_t k) { ok = ok && (k == 0 || fread(p, 1, k, fp) == k); });
-    Fnv f;
-    b.each([&](const void* p, size_t k) { f.add(p, k); });
+    uint64_t sum;
+    if (v1) {
+      Fnv f(kFnvBasisV1);
+      b.each([&](const void* p, size_t k) { f.add(p, k); });
+      sum = f.value();
+    } else {
+      Fnv4 f;
+      b.each([&](const void* p, size_t k) { f.add(p, k); });
+      sum = f.value();
+    }
     if (!ok || fgetc(fp) != EOF) why = "snapshot file truncated or too long";
-    else if (f.h != hd.checksum) why = "snapshot checksum mismatch";
+    else if (sum != hd.checksum) why = "snapshot checksum mismatch";
   }
   fclose(fp);
   if (why.empty()) {

@@ -159,13 +159,20 @@ PBF_API int pbf_estimate_densities(pbf_ctx *ctx, float *rho);
  * snapshot restores a run exactly: loading a snapshot into a context made
  * with the same parameters (and, batched, the same scenes) and stepping on
  * gives the bits of the uninterrupted run.  File layout (little-endian):
- * a 64-byte header {char magic[8] = "PBFSNAP1"; uint32 header_bytes = 64;
+ * a 64-byte header {char magic[8] = "PBFSNAP2"; uint32 header_bytes = 64;
  * uint32 params_bytes = sizeof(pbf_params); int64 n; int32 nscenes;
- * int32 0; double simulate_time; uint64 FNV-1a-64 of the payload;
+ * int32 0; double simulate_time; uint64 checksum of the payload;
  * 16 zero bytes}, then the payload: pbf_params, int64 count[nscenes],
  * float box_lo[3], box_hi[3] per scene, float x[n*3], v[n*3], rho[n]
- * (load order).  Loading checks magic, sizes, checksum, parameters and
- * scenes (PBF_ERR_INVALID on any mismatch) and restores simulate_time. */
+ * (load order).  The checksum runs the FNV-1a-64 step (basis
+ * 14695981039346656037, prime 1099511628211) on 64-bit little-endian
+ * words in four lanes: word l of every 32-byte block of the payload goes to
+ * lane l, the bytes after the last whole block go byte by byte into lane 0,
+ * and the checksum is byte-serial FNV-1a-64 of the four lane values (32
+ * bytes, lane 0 first).  Files with magic "PBFSNAP1" (byte-serial FNV-1a
+ * of the payload from the basis 1469598103934665603) still load.  Loading checks magic, sizes, checksum,
+ * parameters and scenes (PBF_ERR_INVALID on any mismatch) and restores
+ * simulate_time. */
 PBF_API int pbf_save_snapshot(pbf_ctx *ctx, const char *path);
 PBF_API int pbf_load_snapshot(pbf_ctx *ctx, const char *path);
 

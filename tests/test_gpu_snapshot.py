@@ -92,3 +92,60 @@ def test_damaged_or_foreign_snapshots_are_refused(tmp_path):
     with Particles(sc.params) as r:
         with pytest.raises(PbfError):
             r.load_snapshot(str(tmp_path / "missing.pbfsnap"))
+
+
+PRIME = 1099511628211
+MASK = (1 << 64) - 1
+
+
+def fnv1a(data: bytes, h: int) -> int:
+    for b in data:
+        h = ((h ^ b) * PRIME) & MASK
+    return h
+
+
+def fnv4(payload: bytes) -> int:
+    """The version-2 checksum as include/pbf.h states it, in plain Python."""
+    basis = 14695981039346656037
+    lanes = [basis] * 4
+    nblk = len(payload) // 32
+    words = np.frombuffer(payload[:32 * nblk], dtype="<u8").reshape(-1, 4)
+    for w in words:
+        for l in range(4):
+            lanes[l] = ((lanes[l] ^ int(w[l])) * PRIME) & MASK
+    lanes[0] = fnv1a(payload[32 * nblk:], lanes[0])
+    return fnv1a(b"".join(v.to_bytes(8, "little") for v in lanes), basis)
+
+
+def test_snapshot_checksum_is_the_documented_one_and_version_1_loads(tmp_path):
+    """A small scene (odd payload length, so the byte tail is used): the
+    header's checksum equals the documented four-lane FNV-1a; the same
+    payload rewritten as a version-1 file (byte-serial FNV-1a from basis
+    1469598103934665603) loads to the same state."""
+    sc = scenes.dam_block()
+    x, v = sc.x[:333].copy(), sc.v[:333].copy()
+    path = str(tmp_path / "v2.pbfsnap")
+    with Particles(sc.params) as c:
+        c.load(x, v)
+        c.time_step(nsteps=3)
+        c.save_snapshot(path)
+        state = c.read()
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"PBFSNAP2"
+    payload = raw[64:]
+    assert len(payload) % 32 != 0
+    assert int.from_bytes(raw[40:48], "little") == fnv4(payload)
+    v1 = bytearray(raw)
+    v1[:8] = b"PBFSNAP1"
+    v1[40:48] = fnv1a(payload, 1469598103934665603).to_bytes(8, "little")
+    path1 = str(tmp_path / "v1.pbfsnap")
+    open(path1, "wb").write(bytes(v1))
+    with Particles(sc.params) as r:
+        r.load_snapshot(path1)
+        for a, b in zip(r.read(), state):
+            assert bits_equal(a, b)
+    v1[40] ^= 1  # a version-1 file with a wrong checksum is refused
+    open(path1, "wb").write(bytes(v1))
+    with Particles(sc.params) as r:
+        with pytest.raises(PbfError, match="checksum"):
+            r.load_snapshot(path1)
