@@ -614,47 +614,79 @@ namespace {
 int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_t n) {
   if (!xyz || n < 1) return fail(x, PBF_ERR_INVALID, "xyz must be non-NULL and n >= 1");
   if (n >= (int64_t)1 << 31) return fail(x, PBF_ERR_INVALID, "more than 2^31 particles");
-  if (x->nscenes == 1) {
-    x->counts[0] = n;
-  } else {
+  if (x->nscenes > 1) {
     int64_t tot = 0;
     for (auto k : x->counts) tot += k;
     if (tot != n)
       return fail(x, PBF_ERR_INVALID,
                   "n = " + std::to_string(n) + " but the scenes hold " + std::to_string(tot));
   }
-  // Validate every particle against its scene box.
+  // Validate every particle against its scene box and pack the float4
+  // records, on up to 8 host threads over contiguous ranges.  The error
+  // reported is the one a sequential pass would meet first: the lowest
+  // failing particle, and in it the first failing axis (finite, then box).
   std::vector<uint32_t> slot_scene((size_t)n);
-  for (int s = 0; s < x->nscenes; s++) {
-    const SceneGrid& g = x->grids[s];
-    for (int64_t i = x->offsets[s]; i < x->offsets[s] + x->counts[s]; i++) {
-      slot_scene[(size_t)i] = (uint32_t)s;
-      for (int a = 0; a < 3; a++) {
-        const float p = xyz[3 * i + a];
-        const float q = v ? v[3 * i + a] : 0.0f;
-        if (!is_finite_f(p) || !is_finite_f(q))
-          return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(i) + " is not finite");
-        if (p < g.lo[a] || p > g.hi[a])
-          return fail(x, PBF_ERR_OUT_OF_BOX,
-                      "particle " + std::to_string(i) + " lies outside the collision box");
+  std::vector<float4> hp((size_t)n), hv((size_t)n);
+  {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t nthr =
+        n < ((int64_t)1 << 16) ? 1 : (int64_t)std::max(1u, std::min(8u, hw ? hw : 1u));
+    const int64_t per = (n + nthr - 1) / nthr;
+    std::vector<int64_t> bad_at((size_t)nthr, n);
+    std::vector<int> bad_code((size_t)nthr, PBF_OK);
+    auto work = [&](int64_t t) {
+      const int64_t b = t * per, e = std::min(n, b + per);
+      int s = 0;
+      while (s + 1 < x->nscenes && x->offsets[(size_t)s + 1] <= b) s++;
+      for (int64_t i = b; i < e; i++) {
+        while (s + 1 < x->nscenes && x->offsets[(size_t)s + 1] <= i) s++;
+        const SceneGrid& g = x->grids[(size_t)s];
+        slot_scene[(size_t)i] = (uint32_t)s;
+        for (int a = 0; a < 3; a++) {
+          const float p = xyz[3 * i + a];
+          const float q = v ? v[3 * i + a] : 0.0f;
+          int code = PBF_OK;
+          if (!is_finite_f(p) || !is_finite_f(q)) code = PBF_ERR_INVALID;
+          else if (p < g.lo[a] || p > g.hi[a]) code = PBF_ERR_OUT_OF_BOX;
+          if (code != PBF_OK) {
+            bad_at[(size_t)t] = i;
+            bad_code[(size_t)t] = code;
+            return;
+          }
+        }
+        uint32_t id = (uint32_t)i;
+        float idf;
+        memcpy(&idf, &id, 4);
+        hp[(size_t)i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
+        const float r = rho ? rho[i] : 0.0f;
+        hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], r)
+                          : make_float4(0.0f, 0.0f, 0.0f, r);
       }
-    }
+    };
+    std::vector<std::thread> th;
+    for (int64_t t = 1; t < nthr; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& t : th) t.join();
+    int64_t first = n;
+    int code = PBF_OK;
+    for (int64_t t = 0; t < nthr; t++)
+      if (bad_code[(size_t)t] != PBF_OK && bad_at[(size_t)t] < first) {
+        first = bad_at[(size_t)t];
+        code = bad_code[(size_t)t];
+      }
+    if (code == PBF_ERR_INVALID)
+      return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(first) + " is not finite");
+    if (code == PBF_ERR_OUT_OF_BOX)
+      return fail(x, PBF_ERR_OUT_OF_BOX,
+                  "particle " + std::to_string(first) + " lies outside the collision box");
   }
+  // The input is accepted: only now does the context take its size.
+  if (x->nscenes == 1) x->counts[0] = n;
   HIPCHK(x, hipSetDevice(x->device));
   x->n = n;
   x->npad = (n + 63) / 64 * 64;
   int rc = ensure_device(x);
   if (rc != PBF_OK) return rc;
-  std::vector<float4> hp((size_t)n), hv((size_t)n);
-  for (int64_t i = 0; i < n; i++) {
-    uint32_t id = (uint32_t)i;
-    float idf;
-    memcpy(&idf, &id, 4);
-    hp[(size_t)i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
-    const float r = rho ? rho[i] : 0.0f;
-    hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], r)
-                      : make_float4(0.0f, 0.0f, 0.0f, r);
-  }
   pbf::DevState& d = x->d;
   HIPCHK(x, hipMemcpyAsync(d.posA, hp.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
                            x->stream));

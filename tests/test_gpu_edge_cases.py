@@ -134,3 +134,31 @@ def test_list_capacity_at_and_below_the_largest_list():
         with pytest.raises(PbfError) as e:
             c.time_step()
         assert e.value.code == -3
+
+
+@pytest.mark.parametrize("nan_at,out_at,code,first", [
+    (150_000, 170_000, -1, 150_000),
+    (170_000, 150_000, -5, 150_000),
+    (3, 199_999, -1, 3),
+])
+def test_load_reports_the_first_bad_particle(nan_at, out_at, code, first):
+    """Loads of 2^16 particles or more are checked on several host threads;
+    the error is still the one a sequential pass meets first (the lowest bad
+    particle), and a failed load leaves the previous state in place."""
+    sc = scenes.dam_block()
+    p = sc.params
+    rng = np.random.default_rng(9)
+    lo, hi = np.array(p.box_lo, np.float32), np.array(p.box_hi, np.float32)
+    x = rng.uniform(lo, hi, size=(200_000, 3)).astype(np.float32)
+    x[nan_at, 1] = np.nan
+    x[out_at, 2] = hi[2] + 1.0
+    with Particles(p) as c:
+        c.load(sc.x, sc.v)
+        before = c.read()
+        with pytest.raises(PbfError) as e:
+            c.load(x, None)
+        assert e.value.code == code
+        assert f"particle {first} " in str(e.value)
+        after = c.read()
+    for a, b in zip(before, after):
+        assert bits_equal(a, b)
