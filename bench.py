@@ -19,10 +19,11 @@ HBM.  Prints ONE JSON line:
              mean launch time, against 157.3 TFLOP/s as 78.6e12 lane
              operations per second;
 * passes     ms per step, GB/s (state-only and state+list) of every pass;
-* readback, density_field, snapshot, mesh, batched, adjoint  the paths
-             beside the step (positions for the CPU consumers, the
-             marching-cubes density lattice, checkpoint files, the
-             triangle-mesh collider, many small scenes, gradients);
+* readback, estimate_densities, density_field, snapshot, mesh, batched,
+             adjoint  the paths beside the step (positions for the CPU
+             consumers, load-time densities, the marching-cubes density
+             lattice, checkpoint files, the triangle-mesh collider, many
+             small scenes, gradients);
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
              every host core) timed on a few steps of the same scene.
 
@@ -373,6 +374,14 @@ def main() -> int:
     readback = {"metric": "pbf_read of x, v, rho (load order) into new host arrays",
                 "bytes": 28 * sc.n, "ms": read_s * 1e3, "GBs": 28 * sc.n / read_s / 1e9}
     log(f"read back x, v, rho: {read_s * 1e3:.1f} ms")
+    ctx.estimate_densities()
+    t0 = time.perf_counter()
+    rho = ctx.estimate_densities()
+    est_s = time.perf_counter() - t0
+    densities = {"metric": "estimateDensities (load-time densities, self included) into a new "
+                           "host array", "particles": sc.n, "ms": est_s * 1e3,
+                 "mean_density": float(rho.mean())}
+    log(f"estimate_densities: {est_s * 1e3:.1f} ms")
     field = density_field_bench(ctx, sc)
     snap = snapshot_bench(ctx, sc)
     ctx.close()
@@ -458,6 +467,7 @@ def main() -> int:
         "ms_per_step_profiled": wall_profiled / args.steps * 1e3,
         "passes": passes,
         "readback": readback,
+        "estimate_densities": densities,
         "density_field": field,
         "snapshot": snap,
     }
