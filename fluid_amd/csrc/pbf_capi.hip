@@ -11,6 +11,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -325,6 +326,25 @@ int check_ready(Ctx* x) {
   return PBF_OK;
 }
 
+// Runs f(0) .. f(n-1), f(0) on the calling thread and the rest on threads
+// of their own; a part whose thread cannot be started runs on the calling
+// thread, and every started thread is joined before returning.
+template <class F>
+void run_parts(int64_t n, F f) {
+  std::vector<std::thread> th;
+  std::vector<int64_t> inline_parts;
+  for (int64_t t = 1; t < n; t++) {
+    try {
+      th.emplace_back(f, t);
+    } catch (const std::system_error&) {
+      inline_parts.push_back(t);
+    }
+  }
+  f(0);
+  for (int64_t t : inline_parts) f(t);
+  for (auto& t : th) t.join();
+}
+
 // Pinned slabs of kStageSlab floats through which large results return.
 constexpr size_t kStageSlab = (size_t)16 << 20;
 
@@ -375,13 +395,10 @@ hipError_t stage_copy_back(Ctx* x, const float* dsrc, size_t m, float* out) {
     if (e != hipSuccess) return e;
     const float* src = x->stage_host[prev & 1];
     const size_t per = (cnt + nthr - 1) / nthr;
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nthr && t * per < cnt; t++)
-      th.emplace_back([=] {
-        memcpy(out + off + t * per, src + t * per, std::min(per, cnt - t * per) * sizeof(float));
-      });
-    memcpy(out + off, src, std::min(per, cnt) * sizeof(float));
-    for (auto& t : th) t.join();
+    run_parts((int64_t)nthr, [=](int64_t t) {
+      const size_t b = (size_t)t * per;
+      if (b < cnt) memcpy(out + off + b, src + b, std::min(per, cnt - b) * sizeof(float));
+    });
   }
   return hipSuccess;
 }
@@ -663,10 +680,7 @@ int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_
                           : make_float4(0.0f, 0.0f, 0.0f, r);
       }
     };
-    std::vector<std::thread> th;
-    for (int64_t t = 1; t < nthr; t++) th.emplace_back(work, t);
-    work(0);
-    for (auto& t : th) t.join();
+    run_parts(nthr, work);
     int64_t first = n;
     int code = PBF_OK;
     for (int64_t t = 0; t < nthr; t++)
