@@ -48,7 +48,29 @@ def short(name: str) -> str:
     return re.sub(r"\(.*", "", name)
 
 
+# Kernels beside the step that the bench also runs (listed on their own).
+SIDE_OF = [
+    ("k_density_field", "density lattice / points"),
+    ("k_densities", "load-time densities"),
+    ("k_unpermute", "read back (load order)"),
+]
+VALU_PEAK_LANE_OPS = 256 * 128 * 2.4e9  # published FP32 rate, FMA counted once
+
+
+def side_of(kernel: str):
+    # The prediction without movement sorts the committed state for the
+    # density queries; it is not the step's prediction.
+    if kernel.startswith("k_predict") and kernel.endswith("false>"):
+        return "sort of the committed state"
+    for prefix, p in SIDE_OF:
+        if kernel.startswith(prefix):
+            return p
+    return None
+
+
 def pass_of(kernel: str):
+    if side_of(kernel):
+        return None
     for prefix, p in PASS_OF:
         if kernel.startswith(prefix):
             return p
@@ -179,6 +201,27 @@ def counters(args) -> None:
         if tas:
             tot["ta_busy_frac"] = sum(n * f_ for n, f_ in tas) / sum(n for n, _ in tas)
         out["passes"][p] = tot
+    side = []
+    for key in sorted(sq, key=lambda k: -sq[k].get("SQ_INSTS_VALU", 0)):
+        k, grid = key
+        what = side_of(k)
+        if what is None or not dur.get(key):
+            continue
+        rd = fetch.get(key, {}).get("FETCH_SIZE", 0.0) * ff
+        wr = write.get(key, {}).get("WRITE_SIZE", 0.0) * wf
+        v = sq[key].get("SQ_INSTS_VALU", 0.0)
+        us = dur[key]
+        rec = {"what": what, "grid": int(grid), "us": us, "hbm_bytes": rd + wr, "valu_insts": v,
+               "hbm_GBs": (rd + wr) / (us * 1e-6) / 1e9,
+               "valu_frac": v * 64 / (us * 1e-6) / VALU_PEAK_LANE_OPS}
+        out.setdefault("beside", {})[f"{k}[{grid}]"] = rec
+        side.append(f"{k[:28]:28s} {grid:>9s} {us:8.1f} {(rd + wr) / 1e6:9.1f} {rec['hbm_GBs']:8.0f} "
+                    f"{v / 1e6:9.1f} {rec['valu_frac']:6.2f}  {what}")
+    if side:
+        lines += ["", "beside the step (HBM GB/s of the launch; VALU as a fraction of the published "
+                  "FP32 rate, 157.3 TFLOP/s = 78.6e12 lane-ops/s):",
+                  f"{'kernel':28s} {'grid':>9s} {'us':>8s} {'HBM MB':>9s} {'GB/s':>8s} "
+                  f"{'VALU Minst':>9s} {'VALU':>6s}  what"] + side
     lines += ["", "per launch of each pass (HBM bytes = calibrated FETCH + WRITE):"]
     for p, t in out["passes"].items():
         ta_s = f"   TA busy {t['ta_busy_frac']:.2f}" if "ta_busy_frac" in t else ""

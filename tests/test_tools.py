@@ -49,3 +49,9 @@ def test_pass_mapping_covers_the_fused_kernels():
     assert rs.pass_of(rs.short("void pbf::(anonymous namespace)::k_vort_commit_predict<false>(Dev)")) == "vorticity"
     assert rs.pass_of(rs.short("pbf::(anonymous namespace)::k_rank(Dev, float4 const*)")) == "sort"
     assert rs.pass_of("__amd_rocclr_copyBuffer") is None
+    # The prediction without movement belongs to the density queries' sort,
+    # listed beside the step, not to the step's prediction.
+    assert rs.pass_of(rs.short("void pbf::(anonymous namespace)::k_predict<false, true>(Dev)")) == "predict"
+    assert rs.pass_of(rs.short("void pbf::(anonymous namespace)::k_predict<false, false>(Dev)")) is None
+    assert rs.side_of("k_predict<false, false>") == "sort of the committed state"
+    assert rs.side_of("k_density_field<true>") is not None
