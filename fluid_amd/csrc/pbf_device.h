@@ -522,6 +522,17 @@ __device__ __forceinline__ float poly6_sel(const SimConst& c, float r2) {
   return a / c.h9;
 }
 
+// poly6 with the reference's branch (most candidates of a query point lie
+// beyond h, and a wave skips the division when all its lanes do) and the
+// verified short division for the rest: the bits of poly6().
+__device__ __forceinline__ float poly6_br(const SimConst& c, float r2) {
+  if (r2 >= c.h2) return 0.0f;
+  const float t = c.h2 - r2;
+  const float a = c.c1 * ((t * t) * t);
+  if (c.fast_div) return div_seq(a, c.h9, c.r9);
+  return a / c.h9;
+}
+
 __device__ __forceinline__ float3 spiky_sel(const SimConst& c, float rx, float ry, float rz,
                                             float r2) {
   const float rl = c.fast_sqrt ? sqrt_rn(r2) : sqrtf(r2);

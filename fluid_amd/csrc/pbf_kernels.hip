@@ -629,9 +629,7 @@ __device__ __forceinline__ float density_point(float3 p, const Dev& d, const Sim
       for (uint32_t j = s; j < e; j++) {
         const float4 xj = pred[j];
         const float rx = xj.x - p.x, ry = xj.y - p.y, rz = xj.z - p.z;
-        // poly6_sel: the cut-off by selection and the verified short
-        // division; beyond h it adds +0, which changes no bit of the sum.
-        dens = dens + poly6_sel(c, rx * rx + ry * ry + rz * rz);
+        dens = dens + poly6_br(c, rx * rx + ry * ry + rz * rz);
       }
     }
   }
