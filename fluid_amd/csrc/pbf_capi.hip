@@ -380,7 +380,10 @@ hipError_t stage_copy_back(Ctx* x, const float* dsrc, size_t m, float* out) {
   }
   const size_t nslab = (m + kStageSlab - 1) / kStageSlab;
   const unsigned hw = std::thread::hardware_concurrency();
-  const unsigned nthr = std::max(1u, std::min(8u, hw ? hw : 1u));
+  // At least 1M floats (4 MB) per host thread, at most 8 threads.
+  const size_t by_size = std::max<size_t>(1, std::min(m, kStageSlab) >> 20);
+  const unsigned cap = std::max(1u, std::min(8u, hw ? hw : 1u));
+  const unsigned nthr = (unsigned)std::min<size_t>(by_size, cap);
   for (size_t s = 0; s <= nslab; s++) {
     if (s < nslab) {
       const size_t off = s * kStageSlab, cnt = std::min(kStageSlab, m - off);
@@ -1508,11 +1511,19 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
   if (e == hipSuccess && sort) e = pbf::launch_sort_committed(*x, k);
   if (e == hipSuccess)
     e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, x->field_dev);
-  if (e == hipSuccess) e = stage_copy_back(x, x->field_dev, (size_t)m, out);
-  (void)hipFree(dp);
   if (e != hipSuccess) {
+    (void)hipStreamSynchronize(x->stream);
+    (void)hipFree(dp);
     x->field_ready = false;
     return fail(x, PBF_ERR_HIP, std::string("density field: ") + hipGetErrorString(e));
+  }
+  // Small results (a few query points) take a plain copy, large ones the
+  // pinned slabs; either way the stream has finished at return.
+  rc = copy_out(x, x->field_dev, out, (size_t)m * sizeof(float));
+  (void)hipFree(dp);
+  if (rc != PBF_OK) {
+    x->field_ready = false;
+    return rc;
   }
   if (sort) {
     // The sort left the committed state, re-sorted, in posB/velB.
