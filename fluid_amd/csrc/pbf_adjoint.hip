@@ -155,11 +155,18 @@ __global__ __launch_bounds__(kB) void k_to_id(uint32_t n, const float4* __restri
   const uint32_t i = idx();
   if (i < n) dst[__float_as_uint(ids[i].w)] = src[i];
 }
-__global__ __launch_bounds__(kB) void k_from_id(uint32_t n, const float4* __restrict__ ids,
-                                                const float4* __restrict__ src,
-                                                float4* __restrict__ dst) {
+// Two load-order arrays gathered into sorted order (ids: the sorted state's
+// load ids) by one launch.
+__global__ __launch_bounds__(kB) void k_from_id2(uint32_t n, const float4* __restrict__ ids,
+                                                 const float4* __restrict__ src0,
+                                                 float4* __restrict__ dst0,
+                                                 const float4* __restrict__ src1,
+                                                 float4* __restrict__ dst1) {
   const uint32_t i = idx();
-  if (i < n) dst[i] = src[__float_as_uint(ids[i].w)];
+  if (i >= n) return;
+  const uint32_t j = __float_as_uint(ids[i].w);
+  dst0[i] = src0[j];
+  dst1[i] = src1[j];
 }
 __global__ __launch_bounds__(kB) void k_copy4(uint32_t n, const float4* __restrict__ src,
                                               float4* __restrict__ dst) {
@@ -537,11 +544,13 @@ int forward_step(pbf_adj* a, const pbf::SimConst& k, AdjTape* slot) {
     AHIP(a, pbf::launch_predict(*c, k));
     AHIP(a, pbf::launch_scan(*c));
     AHIP(a, pbf::launch_scatter_rank(*c));
-    AHIP(a, pbf::launch_nbr_build(*c, k));
+    // The list build and the first λ pass in one launch, as in the plain
+    // step; neither changes predQ, so the stash of x* can follow.
+    AHIP(a, pbf::launch_nbr_lambda(*c, k, d.predP));
     if (slot) pbf::k_copy4<<<pbf::nblk(n), pbf::kB, 0, c->stream>>>(n, d.predQ, slot->stash);
     for (int it = 0; it < c->p.iterations; it++) {
       const bool last = it == c->p.iterations - 1;
-      AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP, it > 0));
+      if (it > 0) AHIP(a, pbf::launch_lambda(*c, k, d.predQ, d.predP, true));
       AHIP(a, pbf::launch_deltap(*c, k, d.predP, d.predQ, last));
       // The last iteration writes x* and v as XSPH records; the adjoint reads
       // them from predQ and velB.
@@ -579,8 +588,7 @@ int backward_step_l(pbf_adj* a, const pbf::SimConst& k, int t, const AdjTape& T)
   hipStream_t s = c->stream;
   const pbf::SceneGrid g = c->grids[0];
   // dL/dv_new and dL/dx_new in sorted order.
-  pbf::k_from_id<<<nb, pbf::kB, 0, s>>>(n, d.posB, a->vbar, a->vb3);
-  pbf::k_from_id<<<nb, pbf::kB, 0, s>>>(n, d.posB, a->xbar, a->xbn);
+  pbf::k_from_id2<<<nb, pbf::kB, 0, s>>>(n, d.posB, a->vbar, a->vb3, a->xbar, a->xbn);
   pbf::k_adj_vort_a<L><<<nbl, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->vb3, a->etab, a->wb);
   pbf::k_adj_vort_b<L><<<nbl, pbf::kB, 0, s>>>(ad, k, d.predP, d.vort, a->etab, a->wb, a->xbn,
                                               a->xsacc);
