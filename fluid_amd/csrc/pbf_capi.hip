@@ -604,6 +604,7 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   dfree(x->d.meshFlag);
   stage_free(x);
   dfree(x->field_dev);
+  dfree(x->field_pts);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return PBF_OK;
@@ -1495,7 +1496,6 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
       if (!is_finite_f(pts[i])) return fail(x, PBF_ERR_INVALID, "non-finite query point");
   if (m == 0) return PBF_OK;
   const SimConst k = pbf::consts_for(*x, x->p.dt);
-  float* dp = nullptr;
   hipError_t e = hipSuccess;
   if ((size_t)m > x->field_cap) {
     dfree(x->field_dev);
@@ -1503,7 +1503,15 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
     e = dalloc(x->field_dev, (size_t)m);
     if (e == hipSuccess) x->field_cap = (size_t)m;
   }
-  if (e == hipSuccess && !L) e = hipMalloc((void**)&dp, (size_t)m * 3 * sizeof(float));
+  // Query points go to a device buffer the context keeps, so that repeated
+  // small queries (a few points per call) allocate nothing.
+  if (e == hipSuccess && !L && (size_t)m > x->field_pts_cap) {
+    dfree(x->field_pts);
+    x->field_pts_cap = 0;
+    e = dalloc(x->field_pts, 3 * (size_t)m);
+    if (e == hipSuccess) x->field_pts_cap = (size_t)m;
+  }
+  float* dp = L ? nullptr : x->field_pts;
   if (e == hipSuccess && !L)
     e = hipMemcpyAsync(dp, pts, (size_t)m * 3 * sizeof(float), hipMemcpyHostToDevice, x->stream);
   // Sorted once per committed state: later queries reuse the cell table.
@@ -1513,14 +1521,12 @@ static int density_field(Ctx* x, int32_t scene, const float* pts, const pbf::Den
     e = pbf::launch_density_field(*x, k, scene, dp, L, (uint32_t)m, x->field_dev);
   if (e != hipSuccess) {
     (void)hipStreamSynchronize(x->stream);
-    (void)hipFree(dp);
     x->field_ready = false;
     return fail(x, PBF_ERR_HIP, std::string("density field: ") + hipGetErrorString(e));
   }
   // Small results (a few query points) take a plain copy, large ones the
   // pinned slabs; either way the stream has finished at return.
   rc = copy_out(x, x->field_dev, out, (size_t)m * sizeof(float));
-  (void)hipFree(dp);
   if (rc != PBF_OK) {
     x->field_ready = false;
     return rc;

@@ -90,6 +90,8 @@ struct Ctx {
   // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;
   size_t field_cap = 0;
+  float* field_pts = nullptr;        // query points on the device, kept too
+  size_t field_pts_cap = 0;          // points
   float* stage_host[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)

@@ -184,9 +184,10 @@ PBF_API int pbf_load_snapshot(pbf_ctx *ctx, const char *path);
  * evaluated on the cell grid: particles of the 27 cells around p_q in
  * (cell, id) order.  pts: m*3 floats.  Like pbf_estimate_densities it
  * re-sorts the committed state (the positions do not change).  The first
- * call allocates a device buffer of m floats (grown when a later call needs
- * more); results of 4 MB or more return through the pinned slabs of
- * pbf_read.  The context keeps both until pbf_destroy. */
+ * call allocates device buffers for m results and m points (grown when a
+ * later call needs more), so repeated small queries allocate nothing;
+ * results of 4 MB or more return through the pinned slabs of pbf_read.
+ * The context keeps these until pbf_destroy. */
 PBF_API int pbf_density_at(pbf_ctx *ctx, int32_t scene, const float *pts, int64_t m, float *out);
 
 /* The same field on the lattice of marching-cubes corners: point (ix, iy,
