@@ -31,6 +31,9 @@ class Rollout:
         self.n = int(n)
         self.nsteps = int(nsteps)
         self._h = C.c_void_p()
+        # Number of forward passes run so far; the autograd binding records it
+        # so that a backward pass is only taken against the forward it follows.
+        self.forward_count = 0
         cp = params.to_c()
         self._check(lib().pbf_adj_create(C.byref(cp), self.n, self.nsteps, int(checkpoint_every),
                                          C.byref(self._h)))
@@ -40,17 +43,27 @@ class Rollout:
             msg = lib().pbf_adj_last_error(self._h if self._h else None)
             raise PbfError(rc, msg.decode() if msg else "")
 
+    def _rows(self, a, name: str) -> np.ndarray:
+        """a as a contiguous float32 (n, 3) array; any other shape is refused
+        (the library reads and writes n * 3 floats)."""
+        a = np.ascontiguousarray(a, np.float32)
+        if a.shape != (self.n, 3):
+            raise ValueError(f"{name} has shape {a.shape}, expected ({self.n}, 3)")
+        return a
+
     def forward(self, x0, v0, k: float):
-        x0 = np.ascontiguousarray(x0, np.float32).reshape(-1, 3)
-        v0 = np.ascontiguousarray(v0, np.float32).reshape(-1, 3)
+        x0 = self._rows(x0, "x0")
+        v0 = self._rows(v0, "v0")
         xT = np.empty_like(x0)
         vT = np.empty_like(v0)
         self._check(lib().pbf_adj_forward(self._h, _f(x0), _f(v0), float(k), _f(xT), _f(vT)))
+        self.forward_count += 1
         return xT, vT
 
     def backward(self, xbar_T, vbar_T=None):
-        xb = np.ascontiguousarray(xbar_T, np.float32).reshape(-1, 3)
-        vb = None if vbar_T is None else np.ascontiguousarray(vbar_T, np.float32).reshape(-1, 3)
+        """Gradients of the last forward pass."""
+        xb = self._rows(xbar_T, "xbar_T")
+        vb = None if vbar_T is None else self._rows(vbar_T, "vbar_T")
         xb0 = np.empty_like(xb)
         vb0 = np.empty_like(xb)
         kb = C.c_double()
@@ -85,11 +98,17 @@ def rollout_function():
             xT, vT = rollout.forward(x0.detach().cpu().numpy(), v0.detach().cpu().numpy(),
                                      float(k.detach().cpu()))
             ctx.rollout = rollout
+            ctx.forward_id = rollout.forward_count
             dev = v0.device
             return torch.from_numpy(xT).to(dev), torch.from_numpy(vT).to(dev)
 
         @staticmethod
         def backward(ctx, gx, gv):
+            # The rollout's tape holds its last forward pass only.
+            if ctx.rollout.forward_count != ctx.forward_id:
+                raise RuntimeError("the Rollout ran another forward pass after this one; "
+                                   "run backward before the next forward, or use one Rollout "
+                                   "per graph")
             gxn = gx.detach().cpu().numpy() if gx is not None else None
             gvn = gv.detach().cpu().numpy() if gv is not None else None
             if gxn is None:

@@ -331,12 +331,16 @@ int check_ready(Ctx* x) {
 // thread, and every started thread is joined before returning.
 template <class F>
 void run_parts(int64_t n, F f) {
+  // Both vectors are sized before the first thread starts, so nothing below
+  // can throw while a started thread is still joinable.
   std::vector<std::thread> th;
   std::vector<int64_t> inline_parts;
+  th.reserve((size_t)std::max<int64_t>(n, 1));
+  inline_parts.reserve((size_t)std::max<int64_t>(n, 1));
   for (int64_t t = 1; t < n; t++) {
     try {
       th.emplace_back(f, t);
-    } catch (const std::system_error&) {
+    } catch (...) {
       inline_parts.push_back(t);
     }
   }
@@ -698,9 +702,16 @@ int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_
       return fail(x, PBF_ERR_OUT_OF_BOX,
                   "particle " + std::to_string(first) + " lies outside the collision box");
   }
-  // The input is accepted: only now does the context take its size.
-  if (x->nscenes == 1) x->counts[0] = n;
+  // The input is accepted.  From here on the device buffers are replaced or
+  // overwritten, so the context counts as unloaded until every upload has
+  // succeeded: after a failed allocation or copy, pbf_step and pbf_read
+  // return PBF_ERR_STATE instead of running on freed or partial buffers.
   HIPCHK(x, hipSetDevice(x->device));
+  x->loaded = false;
+  x->has_nbrs = false;
+  x->field_ready = false;
+  drop_graph(x);
+  if (x->nscenes == 1) x->counts[0] = n;
   x->n = n;
   x->npad = (n + 63) / 64 * 64;
   int rc = ensure_device(x);

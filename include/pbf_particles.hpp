@@ -62,7 +62,11 @@ class ParticlesT {
   ParticlesT& operator=(const ParticlesT&) = delete;
 
   // -- the reference's interface ------------------------------------------
+  // Appends to the live state: after timeStep the host copy is refreshed
+  // first, so the next upload carries the simulated positions, not the ones
+  // of the last load (src/particles.h:118-120 appends to the live vector).
   void addParticle(Vec3 pos, Vec3 v) {
+    sync();
     x_.push_back((float)pos.x); x_.push_back((float)pos.y); x_.push_back((float)pos.z);
     v_.push_back((float)v.x);   v_.push_back((float)v.y);   v_.push_back((float)v.z);
     rho_.push_back(0.0f);
