@@ -76,6 +76,7 @@ SIGNATURES = {
     "pbf_adj_forward": (C.c_int, [_P, _F, _F, C.c_float, _F, _F]),
     "pbf_adj_backward": (C.c_int, [_P, _F, _F, _F, _F, C.POINTER(C.c_double)]),
     "pbf_adj_stats": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "pbf_adj_replay_check": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "pbf_adj_last_error": (C.c_char_p, [_P]),
     "pbf_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "pbf_step_async": (C.c_int, [_P, C.c_float, C.c_int32]),

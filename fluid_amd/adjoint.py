@@ -76,6 +76,14 @@ class Rollout:
         self._check(lib().pbf_adj_stats(self._h, C.byref(ms), C.byref(contacts)))
         return ms.value, contacts.value
 
+    def replay_mismatches(self) -> int:
+        """Particle checks of the last backward pass whose replayed state
+        differed from the forward pass at a checkpoint (0: the tape the
+        adjoint ran over is the forward trajectory, bit for bit)."""
+        m = C.c_int64()
+        self._check(lib().pbf_adj_replay_check(self._h, C.byref(m)))
+        return m.value
+
     def close(self) -> None:
         if self._h:
             lib().pbf_adj_destroy(self._h)

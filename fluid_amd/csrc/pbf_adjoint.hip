@@ -168,6 +168,26 @@ __global__ __launch_bounds__(kB) void k_from_id2(uint32_t n, const float4* __res
   dst0[i] = src0[j];
   dst1[i] = src1[j];
 }
+// Committed state (any order, load ids in posA.w) against a load-order
+// reference (rx, rv): counts the particles whose x or v differ in any bit.
+__global__ __launch_bounds__(kB) void k_count_diff(uint32_t n, const float4* __restrict__ posA,
+                                                   const float4* __restrict__ velA,
+                                                   const float4* __restrict__ rx,
+                                                   const float4* __restrict__ rv,
+                                                   uint32_t* __restrict__ count) {
+  const uint32_t i = idx();
+  if (i >= n) return;
+  const float4 p = posA[i], v = velA[i];
+  const uint32_t id = __float_as_uint(p.w);
+  const float4 a = rx[id], b = rv[id];
+  const bool same = __float_as_uint(p.x) == __float_as_uint(a.x) &&
+                    __float_as_uint(p.y) == __float_as_uint(a.y) &&
+                    __float_as_uint(p.z) == __float_as_uint(a.z) &&
+                    __float_as_uint(v.x) == __float_as_uint(b.x) &&
+                    __float_as_uint(v.y) == __float_as_uint(b.y) &&
+                    __float_as_uint(v.z) == __float_as_uint(b.z);
+  if (!same) atomicAdd(count, 1u);
+}
 __global__ __launch_bounds__(kB) void k_copy4(uint32_t n, const float4* __restrict__ src,
                                               float4* __restrict__ dst) {
   const uint32_t i = idx();
@@ -453,6 +473,7 @@ struct pbf_adj {
   float k = 0.0f;
   bool have_forward = false;
   float4 *ckx = nullptr, *ckv = nullptr;      // checkpoints, load order
+  float4 *fx = nullptr, *fv = nullptr;        // final state of the forward pass, load order
   std::vector<AdjTape> tape;                  // one slot per step of a segment
   float4 *xbar = nullptr, *vbar = nullptr;    // running adjoints, load order
   float4 *yb = nullptr, *yb2 = nullptr, *xst = nullptr, *xsacc = nullptr, *vb3 = nullptr;
@@ -463,6 +484,7 @@ struct pbf_adj {
   uint32_t* contacts = nullptr;
   double backward_ms = 0.0;
   int64_t last_contacts = 0;
+  int64_t last_replay_diff = 0;  // particles whose replay differed from the forward pass
   std::string err;
 };
 
@@ -492,7 +514,7 @@ void afree(T*& p) {
 }
 
 void free_all(pbf_adj* a) {
-  afree(a->ckx); afree(a->ckv);
+  afree(a->ckx); afree(a->ckv); afree(a->fx); afree(a->fv);
   for (AdjTape& t : a->tape) {
     afree(t.posB); afree(t.predP); afree(t.predQ); afree(t.vort); afree(t.velB);
     afree(t.stash); afree(t.nbr); afree(t.ncnt);
@@ -655,7 +677,7 @@ static int pbf_adj_create_impl(const pbf_params* p, int64_t n, int32_t nsteps, i
   hipError_t e = hipSuccess;
 #define AA(ptr, cnt) \
   if (e == hipSuccess) e = hipMalloc((void**)&ptr, (cnt) * sizeof(*ptr))
-  AA(a->ckx, np * a->nckpt); AA(a->ckv, np * a->nckpt);
+  AA(a->ckx, np * a->nckpt); AA(a->ckv, np * a->nckpt); AA(a->fx, np); AA(a->fv, np);
   // The tape: one slot per step of a checkpoint segment.
   const size_t K = (size_t)pbf::k4_of(*X(a)) * pbf::kChunk;
   a->tape.resize((size_t)every);
@@ -724,6 +746,9 @@ static int pbf_adj_forward_impl(pbf_adj* a, const float* x0, const float* v0, fl
     }
     if ((rc = forward_step(a, k, nullptr)) != PBF_OK) return rc;
   }
+  // The final state, against which the backward pass checks its replay of
+  // the last segment.
+  if ((rc = save_state(a, a->fx, a->fv)) != PBF_OK) return rc;
   rc = pbf_sync(a->ctx);
   if (rc != PBF_OK) return afail(a, rc, pbf_last_error(a->ctx));
   if (xT || vT) {
@@ -764,7 +789,7 @@ static int pbf_adj_backward_impl(pbf_adj* a, const float* xbar_T, const float* v
   AHIP(a, hipEventRecord(e0, s));
   AHIP(a, hipMemcpyAsync(a->xbar, hx.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
   AHIP(a, hipMemcpyAsync(a->vbar, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-  AHIP(a, pbf::launch_fill_u32(a->contacts, 1, 0u, s));
+  AHIP(a, pbf::launch_fill_u32(a->contacts, 2, 0u, s));
   const pbf::SimConst k = pbf::consts_for(*c, c->p.dt);
   int rc = PBF_OK;
   for (int seg = a->nckpt - 1; seg >= 0; seg--) {
@@ -775,6 +800,16 @@ static int pbf_adj_backward_impl(pbf_adj* a, const float* xbar_T, const float* v
       return rc;
     for (int t = 0; t < len; t++)
       if ((rc = forward_step(a, k, &a->tape[(size_t)t])) != PBF_OK) return rc;
+    // The replay must end where the forward pass was at the next checkpoint
+    // (or at its end): every particle's x and v, bit for bit.
+    {
+      const bool last_seg = seg + 1 == a->nckpt;
+      const float4* rx = last_seg ? a->fx : a->ckx + (size_t)(seg + 1) * np;
+      const float4* rv = last_seg ? a->fv : a->ckv + (size_t)(seg + 1) * np;
+      pbf::k_count_diff<<<pbf::nblk(a->n), pbf::kB, 0, s>>>((uint32_t)a->n, c->d.posA, c->d.velA,
+                                                           rx, rv, a->contacts + 1);
+      AHIP(a, hipGetLastError());
+    }
     // Reverse through the segment.
     for (int t = len - 1; t >= 0; t--)
       if ((rc = backward_step(a, k, t0 + t, a->tape[(size_t)t])) != PBF_OK) return rc;
@@ -793,9 +828,10 @@ static int pbf_adj_backward_impl(pbf_adj* a, const float* xbar_T, const float* v
     AHIP(a, hipMemcpy(ks.data(), a->ksteps, ks.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int t = 0; t < a->nsteps; t++) kb += ks[(size_t)t];
   }
-  uint32_t cont = 0;
-  AHIP(a, hipMemcpy(&cont, a->contacts, sizeof cont, hipMemcpyDeviceToHost));
-  a->last_contacts = cont;
+  uint32_t cont[2] = {0, 0};
+  AHIP(a, hipMemcpy(cont, a->contacts, sizeof cont, hipMemcpyDeviceToHost));
+  a->last_contacts = cont[0];
+  a->last_replay_diff = cont[1];
   if (xbar0 || vbar0) {
     AHIP(a, hipMemcpy(hx.data(), a->xbar, n * sizeof(float4), hipMemcpyDeviceToHost));
     AHIP(a, hipMemcpy(hv.data(), a->vbar, n * sizeof(float4), hipMemcpyDeviceToHost));
@@ -823,6 +859,12 @@ PBF_API int pbf_adj_stats(const pbf_adj* a, double* backward_ms, int64_t* wall_c
   if (!a) return afail(nullptr, PBF_ERR_INVALID, "NULL");
   if (backward_ms) *backward_ms = a->backward_ms;
   if (wall_contacts) *wall_contacts = a->last_contacts;
+  return PBF_OK;
+}
+
+PBF_API int pbf_adj_replay_check(const pbf_adj* a, int64_t* mismatched) {
+  if (!a || !mismatched) return afail(nullptr, PBF_ERR_INVALID, "NULL");
+  *mismatched = a->last_replay_diff;
   return PBF_OK;
 }
 

@@ -232,6 +232,12 @@ PBF_API int pbf_adj_backward(pbf_adj *adj, const float *xbar_T, const float *vba
 /* Wall time of the last backward pass (HIP events, ms) and the number of
  * particle-iterations found on a wall during it. */
 PBF_API int pbf_adj_stats(const pbf_adj *adj, double *backward_ms, int64_t *wall_contacts);
+/* The backward pass replays each checkpoint segment and checks that the
+ * replay ends where the forward pass was at the next checkpoint (at its end
+ * for the last segment).  Writes the number of particle-checks of the last
+ * backward pass whose x or v differed in any bit (0: the recorded tape is
+ * the forward pass's trajectory). */
+PBF_API int pbf_adj_replay_check(const pbf_adj *adj, int64_t *mismatched);
 PBF_API const char *pbf_adj_last_error(const pbf_adj *adj);
 
 /* ---- test hooks (inspection of the last step; no counterpart) ---------- */

@@ -36,6 +36,30 @@ def test_two_spheres_4m_step_bit_exact(orc, two_spheres):
     assert 25 < nbar < 40 and nmax <= sc.params.max_neighbors
 
 
+def test_two_spheres_4m_through_the_impact_bit_exact(orc, two_spheres):
+    """The compressed phase of the headline scene: the sphere surfaces start
+    1.0 m apart and close at 4 m/s, so they meet near step 50 (dt 5 ms).
+    55 steps on the GPU, then 6 more on the GPU and in the oracle from the
+    state read back: the same bits, with the spheres in contact (particles
+    within h of the midplane x = 2.5, more neighbours than at rest)."""
+    sc = two_spheres
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.time_step(nsteps=55)
+        x55, v55, _ = c.read()
+        nbar0, nmax0 = c.neighbor_stats()
+        c.time_step(nsteps=6)
+        gx, gv, grho = c.read()
+        nbar, nmax = c.neighbor_stats()
+    mid = np.abs(x55[:, 0] - 2.5) < sc.params.h
+    left = np.arange(sc.n) < sc.n // 2
+    assert (mid & left).any() and (mid & ~left).any()
+    print(f"\nimpact: step 55 mean/max neighbours {nbar0:.2f}/{nmax0}, step 61 {nbar:.2f}/{nmax}; "
+          f"{int(mid.sum())} particles within h of the midplane")
+    ox, ov, orho = orc.step(sc.params, x55, v55, 6, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
+
+
 def test_two_spheres_4m_deterministic_across_runs_and_schedules(two_spheres):
     sc = two_spheres
     outs = []
