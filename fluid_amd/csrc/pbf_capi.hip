@@ -15,11 +15,15 @@
 #include <thread>
 #include <vector>
 
+#include "pbf_host.h"
 #include "pbf_internal.h"
 
 using pbf::Ctx;
 using pbf::SceneGrid;
 using pbf::SimConst;
+namespace host = pbf::host;
+static_assert(sizeof(host::F4) == sizeof(float4) && alignof(float4) % alignof(host::F4) == 0,
+              "host records have the device's float4 layout");
 
 namespace {
 
@@ -326,28 +330,7 @@ int check_ready(Ctx* x) {
   return PBF_OK;
 }
 
-// Runs f(0) .. f(n-1), f(0) on the calling thread and the rest on threads
-// of their own; a part whose thread cannot be started runs on the calling
-// thread, and every started thread is joined before returning.
-template <class F>
-void run_parts(int64_t n, F f) {
-  // Both vectors are sized before the first thread starts, so nothing below
-  // can throw while a started thread is still joinable.
-  std::vector<std::thread> th;
-  std::vector<int64_t> inline_parts;
-  th.reserve((size_t)std::max<int64_t>(n, 1));
-  inline_parts.reserve((size_t)std::max<int64_t>(n, 1));
-  for (int64_t t = 1; t < n; t++) {
-    try {
-      th.emplace_back(f, t);
-    } catch (...) {
-      inline_parts.push_back(t);
-    }
-  }
-  f(0);
-  for (int64_t t : inline_parts) f(t);
-  for (auto& t : th) t.join();
-}
+using host::run_parts;
 
 // Pinned slabs of kStageSlab floats through which large results return.
 constexpr size_t kStageSlab = (size_t)16 << 20;
@@ -647,61 +630,19 @@ int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_
                   "n = " + std::to_string(n) + " but the scenes hold " + std::to_string(tot));
   }
   // Validate every particle against its scene box and pack the float4
-  // records, on up to 8 host threads over contiguous ranges.  The error
-  // reported is the one a sequential pass would meet first: the lowest
-  // failing particle, and in it the first failing axis (finite, then box).
-  std::vector<uint32_t> slot_scene((size_t)n);
-  std::vector<float4> hp((size_t)n), hv((size_t)n);
-  {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int64_t nthr =
-        n < ((int64_t)1 << 16) ? 1 : (int64_t)std::max(1u, std::min(8u, hw ? hw : 1u));
-    const int64_t per = (n + nthr - 1) / nthr;
-    std::vector<int64_t> bad_at((size_t)nthr, n);
-    std::vector<int> bad_code((size_t)nthr, PBF_OK);
-    auto work = [&](int64_t t) {
-      const int64_t b = t * per, e = std::min(n, b + per);
-      int s = 0;
-      while (s + 1 < x->nscenes && x->offsets[(size_t)s + 1] <= b) s++;
-      for (int64_t i = b; i < e; i++) {
-        while (s + 1 < x->nscenes && x->offsets[(size_t)s + 1] <= i) s++;
-        const SceneGrid& g = x->grids[(size_t)s];
-        slot_scene[(size_t)i] = (uint32_t)s;
-        for (int a = 0; a < 3; a++) {
-          const float p = xyz[3 * i + a];
-          const float q = v ? v[3 * i + a] : 0.0f;
-          int code = PBF_OK;
-          if (!is_finite_f(p) || !is_finite_f(q)) code = PBF_ERR_INVALID;
-          else if (p < g.lo[a] || p > g.hi[a]) code = PBF_ERR_OUT_OF_BOX;
-          if (code != PBF_OK) {
-            bad_at[(size_t)t] = i;
-            bad_code[(size_t)t] = code;
-            return;
-          }
-        }
-        uint32_t id = (uint32_t)i;
-        float idf;
-        memcpy(&idf, &id, 4);
-        hp[(size_t)i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
-        const float r = rho ? rho[i] : 0.0f;
-        hv[(size_t)i] = v ? make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], r)
-                          : make_float4(0.0f, 0.0f, 0.0f, r);
-      }
-    };
-    run_parts(nthr, work);
-    int64_t first = n;
-    int code = PBF_OK;
-    for (int64_t t = 0; t < nthr; t++)
-      if (bad_code[(size_t)t] != PBF_OK && bad_at[(size_t)t] < first) {
-        first = bad_at[(size_t)t];
-        code = bad_code[(size_t)t];
-      }
-    if (code == PBF_ERR_INVALID)
-      return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(first) + " is not finite");
-    if (code == PBF_ERR_OUT_OF_BOX)
-      return fail(x, PBF_ERR_OUT_OF_BOX,
-                  "particle " + std::to_string(first) + " lies outside the collision box");
+  // records (host threads; the first failure a sequential pass would meet).
+  std::vector<float> boxes;
+  for (const SceneGrid& g : x->grids) {
+    boxes.insert(boxes.end(), g.lo, g.lo + 3);
+    boxes.insert(boxes.end(), g.hi, g.hi + 3);
   }
+  host::Packed pk;
+  host::validate_pack(xyz, v, rho, n, x->offsets, boxes, 8, &pk);
+  if (pk.code == PBF_ERR_INVALID)
+    return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(pk.first) + " is not finite");
+  if (pk.code == PBF_ERR_OUT_OF_BOX)
+    return fail(x, PBF_ERR_OUT_OF_BOX,
+                "particle " + std::to_string(pk.first) + " lies outside the collision box");
   // The input is accepted.  From here on the device buffers are replaced or
   // overwritten, so the context counts as unloaded until every upload has
   // succeeded: after a failed allocation or copy, pbf_step and pbf_read
@@ -717,13 +658,13 @@ int load_impl(Ctx* x, const float* xyz, const float* v, const float* rho, int64_
   int rc = ensure_device(x);
   if (rc != PBF_OK) return rc;
   pbf::DevState& d = x->d;
-  HIPCHK(x, hipMemcpyAsync(d.posA, hp.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
-                           x->stream));
-  HIPCHK(x, hipMemcpyAsync(d.velA, hv.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice,
-                           x->stream));
+  HIPCHK(x, hipMemcpyAsync(d.posA, pk.pos.data(), (size_t)n * sizeof(float4),
+                           hipMemcpyHostToDevice, x->stream));
+  HIPCHK(x, hipMemcpyAsync(d.velA, pk.vel.data(), (size_t)n * sizeof(float4),
+                           hipMemcpyHostToDevice, x->stream));
   HIPCHK(x, hipMemcpyAsync(d.grids, x->grids.data(), x->grids.size() * sizeof(SceneGrid),
                            hipMemcpyHostToDevice, x->stream));
-  HIPCHK(x, hipMemcpyAsync(d.slotScene, slot_scene.data(), (size_t)n * sizeof(uint32_t),
+  HIPCHK(x, hipMemcpyAsync(d.slotScene, pk.slot_scene.data(), (size_t)n * sizeof(uint32_t),
                            hipMemcpyHostToDevice, x->stream));
   // Scene runs for the persistent schedule, largest first (longest
   // processing time first balances the workgroups).
@@ -1015,66 +956,6 @@ PBF_API int pbf_read(pbf_ctx* c, float* xyz, float* v, float* rho) {
 
 }  // extern "C"
 
-namespace {
-// ---- mesh collider: BVH build on the host --------------------------------
-// Median split on the longest axis of the centroid bounds (ties by index, so
-// the tree is deterministic), leaves of at most 4 triangles.  Nodes are
-// stored in depth-first preorder, so an inner node's left child follows it,
-// and each node records its escape: the first node after its subtree.  The
-// device walks the tree without a stack (a missed box or a finished leaf
-// continues at the escape, a hit inner box at the next node).  Node boxes
-// are padded outwards by 1e-5 of the scene scale so the float slab test is
-// conservative.
-struct MeshBuild {
-  const float* pos;
-  const int32_t* tri;
-  std::vector<uint32_t> order;        // triangle indices, leaf order
-  std::vector<float> cen, lo, hi;     // per triangle: centroid, box
-  std::vector<float4> nodes;          // 2 per node
-  float pad = 0.0f;
-
-  // Returns the node's index; the subtree occupies [index, nodes.size()/2).
-  uint32_t build(uint32_t b, uint32_t e) {
-    const uint32_t ni = (uint32_t)(nodes.size() / 2);
-    nodes.resize(nodes.size() + 2);
-    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = b; i < e; i++) {
-      const uint32_t k = order[i];
-      for (int a = 0; a < 3; a++) {
-        bl[a] = std::min(bl[a], lo[3 * k + a]);
-        bh[a] = std::max(bh[a], hi[3 * k + a]);
-        cl[a] = std::min(cl[a], cen[3 * k + a]);
-        ch[a] = std::max(ch[a], cen[3 * k + a]);
-      }
-    }
-    uint32_t ub = 0;  // inner node: 0; leaf: 2^31 | count << 27 | first triangle
-    if (e - b <= 4) {
-      ub = 0x80000000u | ((e - b) << 27) | b;
-    } else {
-      int ax = 0;
-      for (int a = 1; a < 3; a++)
-        if (ch[a] - cl[a] > ch[ax] - cl[ax]) ax = a;
-      const uint32_t mid = b + (e - b) / 2;
-      std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
-                       [&](uint32_t x, uint32_t y) {
-                         const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
-                         return cx < cy || (cx == cy && x < y);
-                       });
-      build(b, mid);  // node ni + 1
-      build(mid, e);
-    }
-    const uint32_t ua = (uint32_t)(nodes.size() / 2);  // escape
-    float ufa, ufb;
-    memcpy(&ufa, &ua, 4);
-    memcpy(&ufb, &ub, 4);
-    nodes[2 * ni] = make_float4(bl[0] - pad, bl[1] - pad, bl[2] - pad, ufa);
-    nodes[2 * ni + 1] = make_float4(bh[0] + pad, bh[1] + pad, bh[2] + pad, ufb);
-    return ni;
-  }
-};
-}  // namespace
-
 extern "C" {
 
 static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int64_t nv,
@@ -1101,125 +982,32 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
   dfree(x->d.meshFlag);
   x->mesh_ntri = 0;
   if (nt == 0) return PBF_OK;
-  MeshBuild mb;
-  mb.pos = pos;
-  mb.tri = tri;
-  const size_t n = (size_t)nt;
-  mb.order.resize(n);
-  mb.cen.resize(3 * n);
-  mb.lo.resize(3 * n);
-  mb.hi.resize(3 * n);
-  float scale = 0.0f;
-  for (size_t k = 0; k < n; k++) {
-    mb.order[k] = (uint32_t)k;
-    for (int a = 0; a < 3; a++) {
-      const float q0 = pos[3 * (size_t)tri[3 * k] + a], q1 = pos[3 * (size_t)tri[3 * k + 1] + a],
-                  q2 = pos[3 * (size_t)tri[3 * k + 2] + a];
-      mb.lo[3 * k + a] = std::min(q0, std::min(q1, q2));
-      mb.hi[3 * k + a] = std::max(q0, std::max(q1, q2));
-      mb.cen[3 * k + a] = (q0 + q1 + q2) / 3.0f;
-      scale = std::max(scale, std::max(fabsf(mb.lo[3 * k + a]), fabsf(mb.hi[3 * k + a])));
-    }
-  }
-  for (int a = 0; a < 3; a++)
-    scale = std::max(scale, std::max(fabsf(x->grids[0].lo[a]), fabsf(x->grids[0].hi[a])));
-  mb.pad = 1e-5f * std::max(scale, 1e-6f);
-  mb.build(0, (uint32_t)n);
-  // The device walk moves to a larger index at every node, so it ends after
-  // at most nnode nodes; check the links that guarantee it.
-  const uint32_t nnode = (uint32_t)(mb.nodes.size() / 2);
-  for (uint32_t i = 0; i < nnode; i++) {
-    uint32_t esc, b;
-    memcpy(&esc, &mb.nodes[2 * i].w, 4);
-    memcpy(&b, &mb.nodes[2 * i + 1].w, 4);
-    const bool leaf = (b & 0x80000000u) != 0;
-    if (!(esc > i && esc <= nnode) || (!leaf && i + 1 >= nnode) ||
-        (leaf && (((b >> 27) & 7u) == 0 || (b & 0x7ffffffu) + ((b >> 27) & 7u) > n)))
-      return fail(x, PBF_ERR_INVALID, "mesh BVH: inconsistent node links");
-  }
-  // Triangle records in leaf order: p1 (original index bits), e1, e2, n1..n3.
-  std::vector<float4> tv(6 * n);
-  for (size_t i = 0; i < n; i++) {
-    const uint32_t k = mb.order[i];
-    const float* p1 = pos + 3 * (size_t)tri[3 * k];
-    const float* p2 = pos + 3 * (size_t)tri[3 * k + 1];
-    const float* p3 = pos + 3 * (size_t)tri[3 * k + 2];
-    float kf;
-    memcpy(&kf, &k, 4);
-    tv[6 * i] = make_float4(p1[0], p1[1], p1[2], kf);
-    tv[6 * i + 1] = make_float4(p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2], 0.0f);
-    tv[6 * i + 2] = make_float4(p3[0] - p1[0], p3[1] - p1[1], p3[2] - p1[2], 0.0f);
-    for (int j = 0; j < 3; j++) {
-      const float* q = nrm + 3 * (size_t)tri[3 * k + j];
-      tv[6 * i + 3 + j] = make_float4(q[0], q[1], q[2], 0.0f);
-    }
-  }
-  // Cells of the scene grid near a triangle: a segment whose box (grown by
-  // pad) touches none of them skips the BVH walk (mesh_may_hit).  Triangle
-  // boxes are grown by 3 pad here (a hit point lies within pad/2 of its
-  // triangle's box, hit_in_box), far more than the float cell indexing on
-  // the device can err; if marking would visit more than 64 cells per grid
-  // cell, every cell is flagged (the walk always runs).  The flags are then
-  // dilated by one cell.
+  // BVH, triangle records and cell flags on the host (pbf_host.h).
   // PBF_MESH_CELLS=0 leaves the cells out (every segment walks the BVH).
   const pbf::SceneGrid& g = x->grids[0];
+  host::MeshGrid mg;
+  for (int a = 0; a < 3; a++) {
+    mg.lo[a] = g.lo[a];
+    mg.hi[a] = g.hi[a];
+    mg.dims[a] = g.dims[a];
+  }
+  mg.inv_cell = g.inv_cell;
   const char* cells_env = getenv("PBF_MESH_CELLS");
   const bool use_cells = !(cells_env && cells_env[0] == '0');
-  const size_t ncell = use_cells ? (size_t)g.dims[0] * (size_t)g.dims[1] * (size_t)g.dims[2] : 0;
-  std::vector<uint8_t> flag(ncell, 0);
-  if (use_cells) {
-    size_t work = 0;
-    bool all = false;
-    auto cell = [&](double v, int a) {
-      const double f = std::floor((v - (double)g.lo[a]) * (double)g.inv_cell);
-      if (!(f >= 0.0)) return 0;
-      if (!(f < (double)g.dims[a])) return g.dims[a] - 1;
-      return (int)f;
-    };
-    for (size_t k = 0; k < n && !all; k++) {
-      int c0[3], c1[3];
-      size_t cells = 1;
-      for (int a = 0; a < 3; a++) {
-        c0[a] = cell((double)mb.lo[3 * k + a] - 3.0 * mb.pad, a);
-        c1[a] = cell((double)mb.hi[3 * k + a] + 3.0 * mb.pad, a);
-        cells *= (size_t)(c1[a] - c0[a] + 1);
-      }
-      work += cells;
-      if (work > 64 * ncell) {
-        all = true;
-        break;
-      }
-      for (int z = c0[2]; z <= c1[2]; z++)
-        for (int y = c0[1]; y <= c1[1]; y++)
-          for (int xx = c0[0]; xx <= c1[0]; xx++)
-            flag[((size_t)z * g.dims[1] + y) * g.dims[0] + xx] = 1;
-    }
-    if (all) std::fill(flag.begin(), flag.end(), (uint8_t)1);
-    // Dilate by one cell on every axis (three passes of a 3-wide OR): the
-    // device then reads only the start cell of a displacement.
-    const size_t nx = (size_t)g.dims[0], ny = (size_t)g.dims[1], nz = (size_t)g.dims[2];
-    const size_t stride[3] = {1, nx, nx * ny}, dim[3] = {nx, ny, nz};
-    std::vector<uint8_t> tmp(ncell);
-    for (int a = 0; a < 3 && !all; a++) {
-      for (size_t c = 0; c < ncell; c++) {
-        const size_t k = (c / stride[a]) % dim[a];
-        uint8_t v = flag[c];
-        if (k > 0) v |= flag[c - stride[a]];
-        if (k + 1 < dim[a]) v |= flag[c + stride[a]];
-        tmp[c] = v;
-      }
-      flag.swap(tmp);
-    }
-  }
-  hipError_t e = dalloc(x->d.meshTri, tv.size());
-  if (e == hipSuccess) e = dalloc(x->d.meshNode, mb.nodes.size());
+  host::MeshHost mh;
+  const std::string why = host::build_mesh(pos, nrm, tri, nt, mg, use_cells, &mh);
+  if (!why.empty()) return fail(x, PBF_ERR_INVALID, why);
+  const size_t ncell = mh.flags.size();
+  hipError_t e = dalloc(x->d.meshTri, mh.tris.size());
+  if (e == hipSuccess) e = dalloc(x->d.meshNode, mh.nodes.size());
   if (e == hipSuccess && use_cells) e = dalloc(x->d.meshFlag, ncell);
   if (e == hipSuccess && use_cells)
-    e = hipMemcpy(x->d.meshFlag, flag.data(), ncell, hipMemcpyHostToDevice);
+    e = hipMemcpy(x->d.meshFlag, mh.flags.data(), ncell, hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(x->d.meshTri, tv.data(), tv.size() * sizeof(float4), hipMemcpyHostToDevice);
+    e = hipMemcpy(x->d.meshTri, mh.tris.data(), mh.tris.size() * sizeof(float4),
+                  hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(x->d.meshNode, mb.nodes.data(), mb.nodes.size() * sizeof(float4),
+    e = hipMemcpy(x->d.meshNode, mh.nodes.data(), mh.nodes.size() * sizeof(float4),
                   hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     dfree(x->d.meshTri);
@@ -1228,9 +1016,9 @@ static int pbf_set_mesh_impl(pbf_ctx* c, const float* pos, const float* nrm, int
     return fail(x, PBF_ERR_NOMEM, std::string("mesh upload: ") + hipGetErrorString(e));
   }
   x->mesh_ntri = (uint32_t)nt;
-  x->mesh_nnode = nnode;
+  x->mesh_nnode = mh.nnode;
   x->mesh_skin = skin;
-  x->mesh_margin = mb.pad;
+  x->mesh_margin = mh.pad;
   return PBF_OK;
 }
 
@@ -1248,98 +1036,9 @@ PBF_API int pbf_set_mesh(pbf_ctx* c, const float* pos, const float* nrm, int64_t
 }  // extern "C"
 
 namespace {
-// Snapshot file layout (little-endian), see pbf_save_snapshot in pbf.h.
-// Version 2 checksums the payload with Fnv4 (written); version 1 files,
-// checksummed with byte-serial FNV-1a-64, still load.
-constexpr char kSnapMagic[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '2'};
-constexpr char kSnapMagic1[8] = {'P', 'B', 'F', 'S', 'N', 'A', 'P', '1'};
-struct SnapHeader {
-  char magic[8];
-  uint32_t header_bytes;  // sizeof(SnapHeader) = 64
-  uint32_t params_bytes;  // sizeof(pbf_params)
-  int64_t n;
-  int32_t nscenes;
-  int32_t reserved0;
-  double sim_time;
-  uint64_t checksum;      // Fnv4 (version 2) of every byte after the header
-  uint64_t reserved1[2];
-};
-static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
-
-// FNV-1a-64 prime and offset basis; version 1 files used the basis
-// 1469598103934665603 instead.
-constexpr uint64_t kFnvPrime = 1099511628211ull, kFnvBasis = 14695981039346656037ull;
-constexpr uint64_t kFnvBasisV1 = 1469598103934665603ull;
-
-// Byte-serial FNV-1a-64 from basis h0.
-struct Fnv {
-  uint64_t h;
-  explicit Fnv(uint64_t h0 = kFnvBasis) : h(h0) {}
-  void add(const void* p, size_t n) {
-    const unsigned char* b = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * kFnvPrime;
-  }
-  uint64_t value() const { return h; }
-};
-
-// Fnv4: the FNV-1a step on 64-bit little-endian words in four lanes, so the
-// four multiply chains run side by side.  Over the payload as one byte
-// stream: 32-byte block k gives word l (bytes 32k + 8l ..) to lane l; the
-// bytes after the last whole block go byte by byte into lane 0; the value is
-// byte-serial FNV-1a-64 of the four lane values (32 bytes, lane 0 first).
-// Every lane step is a bijection, so any change confined to one lane's words
-// changes the value.
-struct Fnv4 {
-  uint64_t h[4] = {kFnvBasis, kFnvBasis, kFnvBasis, kFnvBasis};
-  unsigned char buf[32];
-  size_t nbuf = 0;
-  void block(const unsigned char* b) {
-    uint64_t w[4];
-    memcpy(w, b, 32);
-    for (int l = 0; l < 4; l++) h[l] = (h[l] ^ w[l]) * kFnvPrime;
-  }
-  void add(const void* p, size_t n) {
-    const unsigned char* b = static_cast<const unsigned char*>(p);
-    while (n > 0 && nbuf > 0) {
-      buf[nbuf++] = *b++;
-      n--;
-      if (nbuf == 32) {
-        block(buf);
-        nbuf = 0;
-      }
-    }
-    for (; n >= 32; n -= 32, b += 32) block(b);
-    while (n > 0) {
-      buf[nbuf++] = *b++;
-      n--;
-    }
-  }
-  uint64_t value() const {
-    uint64_t t = h[0];
-    for (size_t i = 0; i < nbuf; i++) t = (t ^ buf[i]) * kFnvPrime;
-    const uint64_t lanes[4] = {t, h[1], h[2], h[3]};
-    Fnv f;
-    f.add(lanes, sizeof lanes);
-    return f.value();
-  }
-};
-
-// The payload after the header: params, per-scene counts and boxes, x, v, rho.
-struct SnapBody {
-  pbf_params p;
-  std::vector<int64_t> counts;
-  std::vector<float> boxes;  // lo[3], hi[3] per scene
-  std::vector<float> x, v, rho;
-  template <class F>
-  void each(F f) {
-    f(&p, sizeof p);
-    f(counts.data(), counts.size() * sizeof(int64_t));
-    f(boxes.data(), boxes.size() * sizeof(float));
-    f(x.data(), x.size() * sizeof(float));
-    f(v.data(), v.size() * sizeof(float));
-    f(rho.data(), rho.size() * sizeof(float));
-  }
-};
+// Snapshot format and checksum: pbf_host.h.
+using host::SnapBody;
+using host::SnapHeader;
 
 void scene_layout(const Ctx* x, SnapBody* b) {
   b->p = x->p;
@@ -1367,22 +1066,8 @@ static int pbf_save_snapshot_impl(pbf_ctx* c, const char* path) {
   b.v.resize(3 * n);
   b.rho.resize(n);
   if ((rc = pbf_read(c, b.x.data(), b.v.data(), b.rho.data())) != PBF_OK) return rc;
-  SnapHeader hd{};
-  memcpy(hd.magic, kSnapMagic, 8);
-  hd.header_bytes = sizeof(SnapHeader);
-  hd.params_bytes = sizeof(pbf_params);
-  hd.n = x->n;
-  hd.nscenes = x->nscenes;
-  hd.sim_time = x->sim_time;
-  Fnv4 f;
-  b.each([&](const void* p, size_t k) { f.add(p, k); });
-  hd.checksum = f.value();
-  FILE* fp = fopen(path, "wb");
-  if (!fp) return fail(x, PBF_ERR_INVALID, std::string("cannot open ") + path + " for writing");
-  bool ok = fwrite(&hd, sizeof hd, 1, fp) == 1;
-  b.each([&](const void* p, size_t k) { ok = ok && (k == 0 || fwrite(p, 1, k, fp) == k); });
-  ok = (fclose(fp) == 0) && ok;
-  if (!ok) return fail(x, PBF_ERR_INVALID, std::string("write to ") + path + " failed");
+  const std::string why = host::write_snapshot(path, b, x->n, x->sim_time);
+  if (!why.empty()) return fail(x, PBF_ERR_INVALID, why);
   return PBF_OK;
 }
 
@@ -1400,54 +1085,12 @@ static int pbf_load_snapshot_impl(pbf_ctx* c, const char* path) {
   Ctx* x = C(c);
   if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
   if (!path) return fail(x, PBF_ERR_INVALID, "path is NULL");
-  FILE* fp = fopen(path, "rb");
-  if (!fp) return fail(x, PBF_ERR_INVALID, std::string("cannot open ") + path);
-  SnapHeader hd{};
-  const bool head_ok = fread(&hd, sizeof hd, 1, fp) == 1;
-  std::string why;
-  const bool v1 = head_ok && memcmp(hd.magic, kSnapMagic1, 8) == 0;
-  if (!head_ok || (memcmp(hd.magic, kSnapMagic, 8) != 0 && !v1) ||
-      hd.header_bytes != sizeof(SnapHeader))
-    why = "not a snapshot file";
-  else if (hd.params_bytes != sizeof(pbf_params))
-    why = "snapshot written with a different pbf_params layout";
-  else if (hd.nscenes != x->nscenes || hd.n < 1 || hd.n >= ((int64_t)1 << 31))
-    why = "snapshot holds " + std::to_string(hd.nscenes) + " scene(s), the context " +
-          std::to_string(x->nscenes);
   SnapBody b, mine;
-  if (why.empty()) {
-    const size_t n = (size_t)hd.n;
-    b.counts.resize((size_t)hd.nscenes);
-    b.boxes.resize(6 * (size_t)hd.nscenes);
-    b.x.resize(3 * n);
-    b.v.resize(3 * n);
-    b.rho.resize(n);
-    bool ok = true;
-    b.each([&](void* p, size_t k) { ok = ok && (k == 0 || fread(p, 1, k, fp) == k); });
-    uint64_t sum;
-    if (v1) {
-      Fnv f(kFnvBasisV1);
-      b.each([&](const void* p, size_t k) { f.add(p, k); });
-      sum = f.value();
-    } else {
-      Fnv4 f;
-      b.each([&](const void* p, size_t k) { f.add(p, k); });
-      sum = f.value();
-    }
-    if (!ok || fgetc(fp) != EOF) why = "snapshot file truncated or too long";
-    else if (sum != hd.checksum) why = "snapshot checksum mismatch";
-  }
-  fclose(fp);
-  if (why.empty()) {
-    scene_layout(x, &mine);
-    if (memcmp(&b.p, &mine.p, sizeof(pbf_params)) != 0)
-      why = "snapshot written with different parameters";
-    else if (x->nscenes > 1 && b.counts != mine.counts)
-      why = "snapshot scene particle counts differ from the context's";
-    else if (memcmp(b.boxes.data(), mine.boxes.data(), b.boxes.size() * sizeof(float)) != 0)
-      why = "snapshot scene boxes differ from the context's";
-  }
-  if (!why.empty()) return fail(x, PBF_ERR_INVALID, why + ": " + path);
+  scene_layout(x, &mine);
+  SnapHeader hd{};
+  const std::string why = host::read_snapshot(path, mine, &hd, &b);
+  if (!why.empty())
+    return fail(x, PBF_ERR_INVALID, why.rfind("cannot open", 0) == 0 ? why : why + ": " + path);
   const int rc = load_impl(x, b.x.data(), b.v.data(), b.rho.data(), hd.n);
   if (rc != PBF_OK) return rc;
   x->sim_time = hd.sim_time;
