@@ -62,6 +62,8 @@ SIGNATURES = {
     "pbf_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "pbf_density_lattice": (C.c_int, [_P, C.c_int32, _F, _F, C.c_float,
                                       C.POINTER(C.c_int32), _F]),
+    "pbf_step_device_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                                          C.c_int32]),
     "pbf_simulate_time": (C.c_double, [_P]),
     "pbf_count": (C.c_int64, [_P]),
     "pbf_last_error": (C.c_char_p, [_P]),

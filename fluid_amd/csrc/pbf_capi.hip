@@ -13,6 +13,7 @@
 #include <string>
 #include <system_error>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "pbf_host.h"
@@ -1235,6 +1236,191 @@ PBF_API int pbf_density_lattice(pbf_ctx* c, int32_t scene, const float* lo, cons
                                 float step, const int32_t* counts, float* out) {
   try {
     return pbf_density_lattice_impl(c, scene, lo, hi, step, counts, out);
+  } catch (const std::bad_alloc&) {
+    return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
+  } catch (const std::exception& e) {
+    return fail(C(c), PBF_ERR_INVALID, std::string("internal error: ") + e.what());
+  }
+}
+
+// ---- stepping caller-owned device arrays with no allocation -------------
+// Every buffer the step needs beyond the caller's positions and velocities
+// (sorted copies, predictions, cell tables, neighbour lists, flags) lives in
+// one fixed device array.  Each device that runs the library holds its own
+// copy, at its own address, until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for the device.
+namespace {
+
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;  // device -> address of its copy
+
+// Address of this device's copy of the workspace (looked up once per device).
+int scratch_address(Ctx* x, unsigned char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch.find(x->device);
+  if (it == g_scratch.end()) {
+    void* ptr = nullptr;
+    HIPCHK(x, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(x, PBF_ERR_HIP, "no device address for the step workspace");
+    it = g_scratch.emplace(x->device, static_cast<unsigned char*>(ptr)).first;
+  }
+  *out = it->second;
+  return PBF_OK;
+}
+
+// Carves the device state of n particles out of [base, base + cap): the
+// same arrays ensure_device allocates, each aligned to 256 bytes.  With
+// base NULL it only measures; returns the bytes used.
+size_t carve_state(unsigned char* base, size_t n, size_t ncell, size_t K, pbf::DevState* d) {
+  const size_t np = (n + 63) / 64 * 64;
+  size_t off = 0;
+  auto take = [&](auto*& p, size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    p = reinterpret_cast<std::remove_reference_t<decltype(p)>>(base ? base + off : nullptr);
+    off += std::max<size_t>(count, 1) * sizeof(*p);
+  };
+  take(d->posA, np); take(d->posB, np); take(d->velA, np); take(d->velB, np);
+  take(d->predP, np + 64); take(d->predQ, np + 64); take(d->vort, np); take(d->vxsph, np);
+  take(d->xv, 2 * np); take(d->xs3, 3 * (np + 64));
+  take(d->cell, np); take(d->arrival, np); take(d->cellSorted, np); take(d->tmpId, np);
+  take(d->cellCount, ncell + 4); take(d->cellStart, ncell + 4);
+  take(d->blockSums, (ncell + 1023) / 1024 + 1);
+  take(d->nbr, np * K); take(d->ncnt, np); take(d->flags, 64);
+  take(d->grids, 1); take(d->slotScene, np); take(d->runs, 1);
+  d->stage = nullptr;
+  return off;
+}
+
+// Caller arrays (load order, 3 floats per particle) into the committed
+// state: posA = (x, id), velA = (v, 0).  flags[2] / flags[3] receive the
+// lowest index of a non-finite particle / a particle outside the box.
+__global__ __launch_bounds__(256) void k_noalloc_pack(uint32_t n, const float* __restrict__ xyz,
+                                                      const float* __restrict__ v,
+                                                      float4* __restrict__ posA,
+                                                      float4* __restrict__ velA, SceneGrid g,
+                                                      uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float p[3] = {xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2]};
+  const float q[3] = {v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]};
+  bool finite = true, inside = true;
+  for (int a = 0; a < 3; a++) {
+    finite = finite && isfinite(p[a]) && isfinite(q[a]);
+    inside = inside && p[a] >= g.lo[a] && p[a] <= g.hi[a];
+  }
+  if (!finite) atomicMin(flags + 2, i);
+  else if (!inside) atomicMin(flags + 3, i);
+  posA[i] = make_float4(p[0], p[1], p[2], __uint_as_float(i));
+  velA[i] = make_float4(q[0], q[1], q[2], 0.0f);
+}
+
+// Committed state (cell order, ids in posA.w) back into the caller arrays.
+__global__ __launch_bounds__(256) void k_noalloc_unpack(uint32_t n, const float4* __restrict__ posA,
+                                                        const float4* __restrict__ velA,
+                                                        float* __restrict__ xyz,
+                                                        float* __restrict__ v) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = posA[i], q = velA[i];
+  const size_t o = 3 * (size_t)__float_as_uint(p.w);
+  xyz[o] = p.x; xyz[o + 1] = p.y; xyz[o + 2] = p.z;
+  v[o] = q.x; v[o + 1] = q.y; v[o + 2] = q.z;
+}
+
+// Restores the fields of a context that a workspace run borrows.
+struct Borrow {
+  Ctx* x;
+  pbf::DevState d;
+  int64_t n, npad;
+  const float4 *dbg_pred, *dbg_vel;
+  bool has_nbrs;
+  explicit Borrow(Ctx* c)
+      : x(c), d(c->d), n(c->n), npad(c->npad), dbg_pred(c->dbg_pred), dbg_vel(c->dbg_vel),
+        has_nbrs(c->has_nbrs) {}
+  ~Borrow() {
+    x->d = d;
+    x->n = n;
+    x->npad = npad;
+    x->dbg_pred = dbg_pred;
+    x->dbg_vel = dbg_vel;
+    x->has_nbrs = has_nbrs;
+  }
+};
+
+int step_noalloc(Ctx* x, float* xyz, float* v, int64_t n, float dt, int32_t nsteps) {
+  if (!x) return fail(nullptr, PBF_ERR_INVALID, "context is NULL");
+  if (x->nscenes != 1) return fail(x, PBF_ERR_INVALID, "single-scene contexts only");
+  if (!xyz || !v || n < 1) return fail(x, PBF_ERR_INVALID, "xyz and v must be non-NULL and n >= 1");
+  if (n >= (int64_t)1 << 31) return fail(x, PBF_ERR_INVALID, "more than 2^31 particles");
+  if (!is_finite_f(dt) || !(dt > 0.0f) || nsteps < 0)
+    return fail(x, PBF_ERR_INVALID, "dt must be finite and > 0, nsteps >= 0");
+  const size_t K = (size_t)pbf::k4_of(*x) * pbf::kChunk;
+  pbf::DevState probe;
+  const size_t need = carve_state(nullptr, (size_t)n, x->ncell, K, &probe);
+  if (need > kNoAllocBytes) {
+    // The largest n that fits at these parameters.
+    int64_t lo = 0, hi = n;
+    while (lo + 1 < hi) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      (carve_state(nullptr, (size_t)mid, x->ncell, K, &probe) <= kNoAllocBytes ? lo : hi) = mid;
+    }
+    return fail(x, PBF_ERR_INVALID,
+                "n = " + std::to_string(n) + " needs " + std::to_string(need) +
+                    " bytes of the fixed step workspace, which holds " +
+                    std::to_string(kNoAllocBytes) + " (at most " + std::to_string(lo) +
+                    " particles with this grid and max_neighbors)");
+  }
+  HIPCHK(x, hipSetDevice(x->device));
+  unsigned char* base = nullptr;
+  int rc = scratch_address(x, &base);
+  if (rc != PBF_OK) return rc;
+  Borrow keep(x);
+  pbf::DevState d = x->d;  // keeps the mesh collider's buffers
+  carve_state(base, (size_t)n, x->ncell, K, &d);
+  x->d = d;
+  x->n = n;
+  x->npad = (n + 63) / 64 * 64;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  // flags[0]: neighbour overflow; [2], [3]: first bad particle (none: ~0).
+  HIPCHK(x, pbf::launch_fill_u32(d.flags, 2, 0u, x->stream));
+  HIPCHK(x, pbf::launch_fill_u32(d.flags + 2, 2, 0xffffffffu, x->stream));
+  k_noalloc_pack<<<nb, 256, 0, x->stream>>>((uint32_t)n, xyz, v, d.posA, d.velA, x->grids[0],
+                                             d.flags);
+  HIPCHK(x, hipGetLastError());
+  uint32_t bad[4] = {0, 0, 0, 0};
+  HIPCHK(x, hipMemcpyAsync(bad, d.flags, sizeof bad, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  if (bad[2] != 0xffffffffu)
+    return fail(x, PBF_ERR_INVALID, "particle " + std::to_string(bad[2]) + " is not finite");
+  if (bad[3] != 0xffffffffu)
+    return fail(x, PBF_ERR_OUT_OF_BOX,
+                "particle " + std::to_string(bad[3]) + " lies outside the collision box");
+  const SimConst k = pbf::consts_for(*x, dt);
+  for (int s = 0; s < nsteps; s++) {
+    rc = run_step(x, k, 0, nullptr, s > 0, s + 1 < nsteps);
+    if (rc != PBF_OK) return rc;
+  }
+  k_noalloc_unpack<<<nb, 256, 0, x->stream>>>((uint32_t)n, d.posA, d.velA, xyz, v);
+  HIPCHK(x, hipGetLastError());
+  HIPCHK(x, hipMemcpyAsync(bad, d.flags, sizeof bad, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  if (bad[0] != 0)
+    return fail(x, PBF_ERR_OVERFLOW,
+                "a particle had " + std::to_string(bad[0]) + " neighbours; the list capacity is " +
+                    std::to_string(K) + " (max_neighbors " + std::to_string(x->p.max_neighbors) +
+                    ")");
+  return PBF_OK;
+}
+
+}  // namespace
+
+PBF_API int pbf_step_device_noalloc(pbf_ctx* c, float* xyz, float* v, int64_t n, float dt,
+                                    int32_t nsteps) {
+  try {
+    return step_noalloc(C(c), xyz, v, n, dt, nsteps);
   } catch (const std::bad_alloc&) {
     return fail(C(c), PBF_ERR_NOMEM, "host memory exhausted");
   } catch (const std::exception& e) {

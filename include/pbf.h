@@ -200,6 +200,26 @@ PBF_API int pbf_density_at(pbf_ctx *ctx, int32_t scene, const float *pts, int64_
 PBF_API int pbf_density_lattice(pbf_ctx *ctx, int32_t scene, const float lo[3], const float hi[3],
                                 float step, const int32_t counts[3], float *out);
 
+/* Steps caller-owned device arrays without allocating: xyz and v are
+ * DEVICE pointers to n*3 floats each (load order), advanced in place by
+ * nsteps steps of length dt with the physics, box and grid of ctx (a
+ * single-scene context from pbf_create; its own loaded state, if any, and
+ * its simulate_time are left as they are).  The results are the bits
+ * pbf_load + pbf_step_dt + pbf_read give for the same input.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops).  Every
+ * buffer of the step lives in one fixed 64 MiB device array of the library
+ * (each device holds its own copy while the library is loaded); n beyond what it
+ * holds at these parameters (about 55,000 particles at the default
+ * max_neighbors of 256, 115,000 at 96) fails with PBF_ERR_INVALID and a message giving the
+ * limit.  No hipMalloc or hipFree is called.  All calls on one device share
+ * the workspace: calls that overlap on one device (two host threads) race.
+ * Synchronous at return; bad input is reported like pbf_load
+ * (PBF_ERR_INVALID for non-finite values, PBF_ERR_OUT_OF_BOX) before any
+ * step runs, and list overflow like pbf_step_dt. */
+PBF_API int pbf_step_device_noalloc(pbf_ctx *ctx, float *xyz, float *v, int64_t n, float dt,
+                                    int32_t nsteps);
+
 /* Particle::simulate_time of the reference (src/particles.h:109). */
 PBF_API double pbf_simulate_time(const pbf_ctx *ctx);
 PBF_API int64_t pbf_count(const pbf_ctx *ctx);
