@@ -495,6 +495,9 @@ thread_local std::string g_adj_err = "no error";
 int afail(pbf_adj* a, int code, const std::string& m) {
   if (a) a->err = m;
   g_adj_err = m;
+  // Consume the runtime's last error of a failed call (see fail() in
+  // pbf_capi.hip).
+  if (code == PBF_ERR_HIP || code == PBF_ERR_NOMEM) (void)hipGetLastError();
   return code;
 }
 

@@ -33,6 +33,10 @@ thread_local std::string g_err = "no error";
 int fail(Ctx* x, int code, const std::string& msg) {
   if (x) x->err = msg;
   g_err = msg;
+  // A failed runtime call (an allocation that found no memory, say) stays
+  // the thread's last HIP error; consume it here, or the next launch check
+  // (hipGetLastError after a launch) would report it again.
+  if (code == PBF_ERR_HIP || code == PBF_ERR_NOMEM) (void)hipGetLastError();
   return code;
 }
 
@@ -219,6 +223,9 @@ int finish_create(Ctx* x, pbf_ctx** out) {
     return fail(nullptr, PBF_ERR_HIP, m);
   }
   x->device = dev;
+  // PBF_NO_SMALL_SORT keeps the multi-launch scan and sort for small states
+  // too (same bits; for measurement).
+  x->small_sort = getenv("PBF_NO_SMALL_SORT") == nullptr;
   if (verify_math(x) != PBF_OK) {
     std::string m = x->err;
     (void)hipStreamDestroy(x->stream);
@@ -275,9 +282,13 @@ int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr,
              bool predicted = false, bool predict_next = false) {
   pbf::DevState& d = x->d;
   if (!predicted) PASS(PBF_PASS_PREDICT, pbf::launch_predict(*x, c));
-  PASS(PBF_PASS_SCAN, pbf::launch_scan(*x));
   // A prediction fused into the previous step's last pass sits in vort.
-  PASS(PBF_PASS_SORT, pbf::launch_scatter_rank(*x, predicted ? d.vort : nullptr));
+  if (x->small_sort && pbf::sort_in_one_workgroup(*x)) {
+    PASS(PBF_PASS_SORT, pbf::launch_sort_small(*x, predicted ? d.vort : nullptr));
+  } else {
+    PASS(PBF_PASS_SCAN, pbf::launch_scan(*x));
+    PASS(PBF_PASS_SORT, pbf::launch_scatter_rank(*x, predicted ? d.vort : nullptr));
+  }
   x->dbg_pred = d.predQ;
   x->dbg_vel = d.velB;
   if (stage == PBF_STAGE_PREDICT) return PBF_OK;
@@ -503,7 +514,7 @@ PBF_API void pbf_params_default(pbf_params* p) {
   p->box_hi[0] = 1.0f;  p->box_hi[1] = 1.49f; p->box_hi[2] = 1.0f;
   p->wall_stop_mask = (1u << 3) | (1u << 5);
   p->cell_scale = 1.0f + 1.0f / 1024.0f;
-  p->max_neighbors = 96;
+  p->max_neighbors = 256;
   p->poly6_coef = 1.56668147106f;
   p->spiky_coef = 4.774648292756860f;
   p->eps_d = 0.00000000001f;

@@ -69,6 +69,18 @@ __device__ __forceinline__ float ipow(float b, int n) {
   return r;
 }
 
+// intpow with the exponent known at compile time (NT >= 0), the same
+// products in the same order; NT < 0 takes n at run time.
+template <int NT>
+__device__ __forceinline__ float ipow_n(float b, int n) {
+  if (NT < 0) return ipow(b, n);
+  if (NT == 0) return 1.0f;
+  float r = b;
+#pragma unroll
+  for (int k = 1; k < (NT > 0 ? NT : 1); k++) r = r * b;
+  return r;
+}
+
 // std::min / std::max as the reference's per-axis clamp uses them.
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }

@@ -487,8 +487,10 @@ __device__ __forceinline__ float3* xv_record(const Dev& d, uint32_t i) {
 
 // newtonStepUpdatePosition (src/particles.cpp:206-213) with s_corr (:195)
 // and clamp (:51-84).  in = (x*, lambda), out = (x*', 0).  The last
-// iteration also forms v = (x* - x) / dt (updateVelocity, :215-217).
-template <bool LAST>
+// iteration also forms v = (x* - x) / dt (updateVelocity, :215-217).  NT >= 0
+// is the tensile exponent N known at compile time (the scenes' 4): the
+// same products, unrolled; NT < 0 reads it from c.
+template <bool LAST, int NT = -1>
 __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimConst& c,
                                            const SceneGrid& g, const float4* __restrict__ in,
                                            float4* __restrict__ out) {
@@ -501,7 +503,7 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
         const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         const float r2 = rx * rx + ry * ry + rz * rz;
         const float w = poly6_sel(c, r2);
-        const float s = c.negk * ipow(w * c.S, c.n_tensile);
+        const float s = c.negk * ipow_n<NT>(w * c.S, c.n_tensile);
         const float3 gr = spiky_sel(c, rx, ry, rz, r2);
         const float sc = (xi.w + xj.w) + s;
         dx = dx + sc * gr.x;
@@ -703,11 +705,11 @@ __global__ __launch_bounds__(kBlock) void k_lambda(Dev d, SimConst c, const floa
   if (i < d.n) lambda_one<PACKED>(i, d, c, in, out);
 }
 
-template <bool MULTI, bool LAST>
+template <bool MULTI, bool LAST, int NT>
 __global__ __launch_bounds__(kBlock) void k_deltap(Dev d, SimConst c, SceneGrid g0,
                                                    const float4* in, float4* out) {
   const uint32_t i = gather_index();
-  if (i < d.n) deltap_one<LAST>(i, d, c, grid_of<MULTI>(i, g0, d), in, out);
+  if (i < d.n) deltap_one<LAST, NT>(i, d, c, grid_of<MULTI>(i, g0, d), in, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_xsph(Dev d, SimConst c, float4* out) {
@@ -850,6 +852,22 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const S
   }
 }
 
+// Scan, scatter and rank of a small state in one workgroup: the three
+// passes that otherwise take four or five launches, with workgroup barriers
+// between them (every thread of the workgroup shares one CU's cache, so a
+// barrier makes the phase's global writes visible).  The same per-particle
+// code as the flat launches: the same bits.
+constexpr int kSortSmallBlock = 1024;
+__global__ __launch_bounds__(kSortSmallBlock) void k_sort_small(Dev d, const float4* pred,
+                                                                uint32_t ncell) {
+  __shared__ uint32_t wsum[kSortSmallBlock / 64 + 1];
+  block_scan<kSortSmallBlock>(d.cellCount, d.cellStart, ncell, 0u, wsum);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < d.n; i += kSortSmallBlock) scatter_one(i, d);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < d.n; i += kSortSmallBlock) rank_one(i, d, pred);
+}
+
 // ------------------------------------------------------------ read-back --
 __global__ __launch_bounds__(kBlock) void k_unpermute(uint32_t n, const float4* __restrict__ ids,
                                                       const float4* __restrict__ src, int first,
@@ -981,6 +999,16 @@ hipError_t launch_scan(Ctx& x) {
   return launch_scan_u32(x.d.cellCount, x.d.cellStart, x.ncell, x.d.blockSums, x.stream);
 }
 
+bool sort_in_one_workgroup(const Ctx& x) {
+  return x.n <= (int64_t)kSortSmallMax && x.ncell <= kSortSmallCells;
+}
+
+hipError_t launch_sort_small(Ctx& x, const float4* pred) {
+  const Dev d = dev_of(x);
+  k_sort_small<<<1, kSortSmallBlock, 0, x.stream>>>(d, pred ? pred : d.predP, x.ncell);
+  return hipGetLastError();
+}
+
 hipError_t launch_scatter_rank(Ctx& x, const float4* pred) {
   const Dev d = dev_of(x);
   k_scatter<<<blocks_for(x.n), kBlock, 0, x.stream>>>(d);
@@ -1014,20 +1042,29 @@ hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* ou
   return hipGetLastError();
 }
 
-hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last) {
-  const Dev d = dev_of(x);
+template <int NT>
+static void launch_deltap_nt(const Ctx& x, const Dev& d, const SimConst& c, const float4* in,
+                             float4* out, bool last) {
   const unsigned nb = blocks_for(x.n);
   if (multi(x)) {
     if (last)
-      k_deltap<true, true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<true, true, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
     else
-      k_deltap<true, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<true, false, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
   } else {
     if (last)
-      k_deltap<false, true><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<false, true, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
     else
-      k_deltap<false, false><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<false, false, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
   }
+}
+
+hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* out, bool last) {
+  const Dev d = dev_of(x);
+  if (c.n_tensile == 4)
+    launch_deltap_nt<4>(x, d, c, in, out, last);
+  else
+    launch_deltap_nt<-1>(x, d, c, in, out, last);
   return hipGetLastError();
 }
 

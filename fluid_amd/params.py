@@ -36,7 +36,7 @@ class Params:
     box_hi: tuple = (1.0, 1.49, 1.0)
     wall_stop_mask: int = WALL_Y_HI | WALL_Z_HI
     cell_scale: float = 1.0 + 1.0 / 1024.0
-    max_neighbors: int = 96
+    max_neighbors: int = 256
     poly6_coef: float = POLY6_LITERAL
     spiky_coef: float = SPIKY_LITERAL
     eps_d: float = EPS_D

@@ -9,13 +9,26 @@ checkpoints every 10 steps, loss = mean |x_T - target|^2.
   adjoint kernels read is the forward trajectory.  Two backward passes give
   the same bits.
 * Against FP64 central differences of the oracle (grid mode, step 1e-6)
-  along dL/dv0 and along one random unit direction: within 1e-3 relative
-  (the random direction against |dL/dv0|) at 10 and 20 steps.  Beyond
-  that the rollout is chaotic (the FP32 and FP64 trajectories of the oracle
-  part by 0.1 m at 40 steps and 0.23 m at 60), and the FP64 central
-  differences themselves do not converge: along a random direction they
-  read -2.7e-3, 3.4e-2, -0.21 and 2.4 at steps 1e-4, 1e-5, 1e-6 and 1e-7
-  over 60 steps.  There the comparison is printed, not asserted.
+  along dL/dv0 and along one random unit direction, within 1e-3 relative
+  (the random direction against |dL/dv0|): asserted at 10 steps (measured
+  2.8e-6 along the gradient, 2.9e-7 along the random direction).  From step
+  6 on, the tensile instability brings pairs of particles within 3.5e-4 h
+  of each other (19 pairs closer than 0.01 h at step 20), where the spiky
+  gradient's direction r/|r| turns over displacements of ~1e-5 m: the
+  derivative there is exact but holds only within that radius, and the
+  FP32 and FP64 trajectories put those pairs at different distances.  At
+  20 steps 57-71 % of |dL/dv0|^2 sits on ten particles next to such pairs,
+  and FP32 central differences along the gradient read 4.5e-4 to 6.7e-4
+  for steps 0.1 to 0.001 against the adjoint's 5.7e-3 (vorticity
+  confinement off); with confinement the loss is rougher still (its
+  direction eta/|eta| turns fast where |eta| is small).  At 40 and 60 steps
+  FP64 central differences do not converge (along a random direction at
+  60 steps: -7.5e-3, 1.2e-2, -1.7e-2, -0.46, -0.67 for steps 1e-3 to 1e-7).
+  So the comparison at 20, 40 and 60 steps is printed, not asserted.
+* The checkpoint machinery at the full horizon: 60 steps with checkpoints
+  every 10, 7 and 60 steps give the same gradients bit for bit (each
+  segment's replay recomputes the forward steps exactly and the adjoint
+  kernels are deterministic).
 """
 import numpy as np
 import pytest
@@ -26,7 +39,10 @@ from fluid_amd.adjoint import Rollout
 pytestmark = pytest.mark.gpu
 
 HORIZONS = (10, 20, 40, 60)
-CHECKED = (10, 20)
+# (steps, constants) pairs whose FD comparison is asserted; "novort": the
+# workload's constants with vorticity confinement off.
+CHECKED = ((10, "scene"),)
+RUNS = [(T, "scene") for T in HORIZONS] + [(20, "novort")]
 TOL = 1e-3
 H = 1e-6
 
@@ -37,8 +53,9 @@ def full():
     assert sc.n == 32768 and sc.steps == 60
     k = float(np.float32(sc.params.k_tensile))
     runs = {}
-    for T in HORIZONS:
-        r = Rollout(sc.params, sc.n, T, checkpoint_every=10)
+    for T, var in RUNS:
+        p = sc.params if var == "scene" else sc.params.replace(vort_eps=0.0)
+        r = Rollout(p, sc.n, T, checkpoint_every=10)
         xT, vT = r.forward(sc.x, sc.v, k)
         xbar = ((2.0 / sc.n) * (xT.astype(np.float64) - target)).astype(np.float32)
         _, vb0, kb = r.backward(xbar)
@@ -46,8 +63,8 @@ def full():
         _, contacts = r.stats()
         _, vb0_again, kb_again = r.backward(xbar)
         r.close()
-        runs[T] = dict(xT=xT, vT=vT, vb0=vb0, kb=kb, mism=mism, contacts=contacts,
-                       again=(vb0_again, kb_again))
+        runs[(T, var)] = dict(xT=xT, vT=vT, vb0=vb0, kb=kb, mism=mism, contacts=contacts,
+                              again=(vb0_again, kb_again), params=p)
     return sc, target.astype(np.float64), k, runs
 
 
@@ -58,14 +75,15 @@ def test_forward_bit_exact_with_oracle(orc, full):
     for T in HORIZONS:
         x, v, _ = orc.step(sc.params, x, v, T - done, orc.NBR_GRID)
         done = T
-        assert np.array_equal(runs[T]["xT"].view(np.uint32), x.view(np.uint32)), T
-        assert np.array_equal(runs[T]["vT"].view(np.uint32), v.view(np.uint32)), T
+        r = runs[(T, "scene")]
+        assert np.array_equal(r["xT"].view(np.uint32), x.view(np.uint32)), T
+        assert np.array_equal(r["vT"].view(np.uint32), v.view(np.uint32)), T
 
 
 def test_replay_is_the_forward_and_backward_repeats(full):
     _, _, _, runs = full
-    for T in HORIZONS:
-        r = runs[T]
+    for key, r in runs.items():
+        T = key
         assert r["mism"] == 0, T
         assert r["contacts"] == 0, T
         vb, kb = r["again"]
@@ -74,31 +92,48 @@ def test_replay_is_the_forward_and_backward_repeats(full):
         assert np.isfinite(r["vb0"]).all() and np.isfinite(r["kb"]), T
 
 
-def _loss64(orc, sc, target, v0, k, T):
-    p = sc.params.replace(k_tensile=k)
+def _loss64(orc, sc, target, v0, k, T, params):
+    p = params.replace(k_tensile=k)
     x, _, _ = orc.step(p.as_dict(), sc.x.astype(np.float64), v0, T, dtype=np.float64)
     return float(np.mean(np.sum((x - target) ** 2, axis=1)))
 
 
-def _fd(orc, sc, target, k, T, d):
+def _fd(orc, sc, target, k, T, d, params):
     v0 = sc.v.astype(np.float64)
-    return (_loss64(orc, sc, target, v0 + H * d, k, T) -
-            _loss64(orc, sc, target, v0 - H * d, k, T)) / (2 * H)
+    return (_loss64(orc, sc, target, v0 + H * d, k, T, params) -
+            _loss64(orc, sc, target, v0 - H * d, k, T, params)) / (2 * H)
 
 
-@pytest.mark.parametrize("T", HORIZONS)
-def test_gradient_against_fp64_central_differences(orc, full, T):
+@pytest.mark.parametrize("T,var", RUNS)
+def test_gradient_against_fp64_central_differences(orc, full, T, var):
     sc, target, k, runs = full
-    g = runs[T]["vb0"].astype(np.float64)
+    run = runs[(T, var)]
+    g = run["vb0"].astype(np.float64)
     gn = float(np.linalg.norm(g))
-    fd_g = _fd(orc, sc, target, k, T, g / gn)
+    fd_g = _fd(orc, sc, target, k, T, g / gn, run["params"])
     d = np.random.default_rng(17).standard_normal(g.shape)
     d /= np.linalg.norm(d)
-    fd_r = _fd(orc, sc, target, k, T, d)
+    fd_r = _fd(orc, sc, target, k, T, d, run["params"])
     ad_r = float(np.sum(g * d))
     rel_g = abs(fd_g - gn) / abs(fd_g)
     rel_r = abs(fd_r - ad_r) / gn
-    print(f"\n[32768 particles, {T} steps] |dL/dv0| {gn:.6e}: along it fd {fd_g:.6e} rel {rel_g:.2e}; "
-          f"random: fd {fd_r:.6e} adjoint {ad_r:.6e} |err|/|grad| {rel_r:.2e}; dL/dk {runs[T]['kb']:.6e}")
-    if T in CHECKED:
+    print(f"\n[32768 particles, {T} steps, {var}] |dL/dv0| {gn:.6e}: along it fd {fd_g:.6e} "
+          f"rel {rel_g:.2e}; random: fd {fd_r:.6e} adjoint {ad_r:.6e} |err|/|grad| {rel_r:.2e}; "
+          f"dL/dk {run['kb']:.6e}")
+    if (T, var) in CHECKED:
         assert rel_g <= TOL and rel_r <= TOL
+
+
+def test_checkpoint_spacing_gives_the_same_bits(full):
+    sc, target, k, runs = full
+    ref = runs[(60, "scene")]
+    xbar = ((2.0 / sc.n) * (ref["xT"].astype(np.float64) - target)).astype(np.float32)
+    for every in (7, 60):
+        r = Rollout(sc.params, sc.n, 60, checkpoint_every=every)
+        xT, _ = r.forward(sc.x, sc.v, k)
+        _, vb0, kb = r.backward(xbar)
+        assert r.replay_mismatches() == 0
+        r.close()
+        assert np.array_equal(xT.view(np.uint32), ref["xT"].view(np.uint32)), every
+        assert np.array_equal(vb0.view(np.uint32), ref["vb0"].view(np.uint32)), every
+        assert kb == ref["kb"], every

@@ -162,3 +162,35 @@ def test_load_reports_the_first_bad_particle(nan_at, out_at, code, first):
         after = c.read()
     for a, b in zip(before, after):
         assert bits_equal(a, b)
+
+
+def test_failed_allocation_leaves_the_context_unloaded():
+    """A load whose device buffers cannot be allocated (20M particles with
+    a 4096-entry list each: 328 GB of lists on a 288 GB device) fails with
+    PBF_ERR_NOMEM and leaves the context unloaded: stepping and reading
+    then fail with PBF_ERR_STATE instead of touching freed buffers, and a
+    later load of a small scene works again."""
+    from fluid_amd import params as P
+
+    p = P.reference_params().replace(max_neighbors=4096)
+    rng = np.random.default_rng(1)
+    small = rng.uniform([-0.9, 0.1, -0.9], [0.9, 1.3, 0.9], (500, 3)).astype(np.float32)
+    with Particles(p) as c:
+        c.load(small)
+        c.time_step(nsteps=1)
+        n = 20_000_000
+        big = rng.uniform([-0.9, 0.1, -0.9], [0.9, 1.3, 0.9], (n, 3)).astype(np.float32)
+        with pytest.raises(PbfError) as e:
+            c.load(big)
+        assert e.value.code == -6  # PBF_ERR_NOMEM
+        del big
+        with pytest.raises(PbfError) as e:
+            c.time_step(nsteps=1)
+        assert e.value.code == -4  # PBF_ERR_STATE
+        with pytest.raises(PbfError) as e:
+            c.read()
+        assert e.value.code == -4
+        c.load(small)
+        c.time_step(nsteps=1)
+        x, _, _ = c.read()
+        assert x.shape == (500, 3) and np.isfinite(x).all()

@@ -199,6 +199,31 @@ def test_large_grid_multi_block_scan(orc):
     assert all(np.array_equal(ids[i, : counts[i]], lg[og[i]:og[i + 1]]) for i in range(len(xs)))
 
 
+def test_small_state_single_workgroup_sort_over_several_scan_chunks(orc):
+    """4,096 particles over a 27^3-cell grid: the scan, scatter and rank run
+    in one workgroup (the scan in five 4,096-cell chunks); 5 steps bit-exact
+    with the oracle, and with the multi-launch path forced in a context
+    created under PBF_NO_SMALL_SORT, the same bits."""
+    import os
+
+    sc = scenes.dam_block()
+    p = sc.params.replace(box_hi=(2.7, 2.7, 2.7))
+    xg, vg, _ = orc.step(p, sc.x, sc.v, 5, orc.NBR_GRID)
+    outs = []
+    for env in (None, "1"):
+        if env:
+            os.environ["PBF_NO_SMALL_SORT"] = env
+        try:
+            with ctx_for(p, sc.x, sc.v) as c:
+                assert np.prod(c.grid_dims()) == 27 ** 3
+                c.time_step(nsteps=5)
+                outs.append(c.read())
+        finally:
+            os.environ.pop("PBF_NO_SMALL_SORT", None)
+    for gx, gv, _ in outs:
+        assert bits_equal(gx, xg) and bits_equal(gv, vg)
+
+
 def test_two_sphere_collision_bit_exact(orc):
     """A small two-sphere impact (v_x = +-2) with the c2 constants."""
     sc = scenes.two_spheres(per_sphere=3000, spacing=0.04, steps=5)
