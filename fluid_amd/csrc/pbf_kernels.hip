@@ -441,6 +441,85 @@ __device__ __forceinline__ void nbr_one(uint32_t i, const Dev& d, float h2, cons
   d.ncnt[i] = cnt;
 }
 
+// ---- per-neighbour terms of the list passes ----------------------------------
+// Each pass adds, for every neighbour j in list order, a few terms to sums
+// that start at +0.  The terms are written once here and used by both the
+// per-lane loops and the wide (wave-per-particle) passes, so both add the
+// same values in the same order.
+
+// newtonStepCalculateLambda (src/particles.cpp:189-197): W, g/rho0 (3),
+// |g/rho0|^2.
+__device__ __forceinline__ void lambda_terms(const SimConst& c, const float3& xi,
+                                             const float3& xj, float* t) {
+  const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float w = poly6_sel(c, r2);
+  float3 gr = spiky_sel(c, rx, ry, rz, r2);
+  gr.x = gr.x * c.inv_rho0;
+  gr.y = gr.y * c.inv_rho0;
+  gr.z = gr.z * c.inv_rho0;
+  t[0] = w;
+  t[1] = gr.x;
+  t[2] = gr.y;
+  t[3] = gr.z;
+  t[4] = gr.x * gr.x + gr.y * gr.y + gr.z * gr.z;
+}
+
+// newtonStepUpdatePosition (src/particles.cpp:208-210) with s_corr (:195):
+// (lambda_i + lambda_j + s_ij) g_ij (3).  x .w carries lambda.
+template <int NT>
+__device__ __forceinline__ void deltap_terms(const SimConst& c, const float4& xi,
+                                             const float4& xj, float* t) {
+  const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float w = poly6_sel(c, r2);
+  const float s = c.negk * ipow_n<NT>(w * c.S, c.n_tensile);
+  const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+  const float sc = (xi.w + xj.w) + s;
+  t[0] = sc * gr.x;
+  t[1] = sc * gr.y;
+  t[2] = sc * gr.z;
+}
+
+// calculateVorticityApplyXSPHViscosity (src/particles.cpp:221-231): the
+// vorticity (v_j - v_i) x g (3), the viscosity (v_j - v_i) W (3), W.
+__device__ __forceinline__ void xsph_terms(const SimConst& c, const float3& xi, const float3& vi,
+                                           const float3& xj, const float3& vj, float* t) {
+  const float u0 = vj.x - vi.x, u1 = vj.y - vi.y, u2 = vj.z - vi.z;
+  const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+  t[0] = u1 * gr.z - u2 * gr.y;
+  t[1] = u2 * gr.x - u0 * gr.z;
+  t[2] = u0 * gr.y - u1 * gr.x;
+  const float w = poly6_sel(c, r2);
+  t[3] = u0 * w;
+  t[4] = u1 * w;
+  t[5] = u2 * w;
+  t[6] = w;
+}
+
+// applyVorticity (src/particles.cpp:238-240): |omega_j| g_ij (3); x .w
+// carries |omega|.
+__device__ __forceinline__ void vort_terms(const SimConst& c, const float4& xi, const float4& xj,
+                                           float* t) {
+  const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+  const float r2 = rx * rx + ry * ry + rz * rz;
+  const float3 gr = spiky_sel(c, rx, ry, rz, r2);
+  t[0] = xj.w * gr.x;
+  t[1] = xj.w * gr.y;
+  t[2] = xj.w * gr.z;
+}
+
+// lambda from the sums of newtonStepCalculateLambda (src/particles.cpp:
+// 198-204): C = rho/rho0 - 1, D = sum |g/rho0|^2 + |G|^2, lambda = -C/(D+eps).
+__device__ __forceinline__ float lambda_of(const SimConst& c, float dens, float denom, float Gx,
+                                           float Gy, float Gz) {
+  const float ci = dens / c.rho0 - 1.0f;
+  denom = denom + (Gx * Gx + Gy * Gy + Gz * Gz);
+  return -ci / (denom + c.eps);
+}
+
 // newtonStepCalculateLambda (src/particles.cpp:185-204).  out = (x*, lambda).
 // x* comes from `in` (after the sort), or with PACKED from the 12-byte
 // records that the previous delta-p pass wrote (then `in` is not read).
@@ -460,23 +539,15 @@ __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimCo
         return make_float3(q.x, q.y, q.z);
       },
       [&](uint32_t, const float3& xj) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float w = poly6_sel(c, r2);
-        float3 gr = spiky_sel(c, rx, ry, rz, r2);
-        dens = dens + w;
-        gr.x = gr.x * c.inv_rho0;
-        gr.y = gr.y * c.inv_rho0;
-        gr.z = gr.z * c.inv_rho0;
-        Gx = Gx + gr.x;
-        Gy = Gy + gr.y;
-        Gz = Gz + gr.z;
-        denom = denom + (gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+        float t[5];
+        lambda_terms(c, xi, xj, t);
+        dens = dens + t[0];
+        Gx = Gx + t[1];
+        Gy = Gy + t[2];
+        Gz = Gz + t[3];
+        denom = denom + t[4];
       });
-  const float ci = dens / c.rho0 - 1.0f;
-  denom = denom + (Gx * Gx + Gy * Gy + Gz * Gz);
-  const float lambda = -ci / (denom + c.eps);
-  out[i] = make_float4(xi.x, xi.y, xi.z, lambda);
+  out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, dens, denom, Gx, Gy, Gz));
 }
 
 // (x*, v) record of sorted particle i after the solve: 6 consecutive floats,
@@ -484,6 +555,11 @@ __device__ __forceinline__ void lambda_one(uint32_t i, const Dev& d, const SimCo
 __device__ __forceinline__ float3* xv_record(const Dev& d, uint32_t i) {
   return reinterpret_cast<float3*>(d.xv) + 2 * (size_t)i;
 }
+
+template <bool LAST>
+__device__ __forceinline__ void deltap_finish(uint32_t i, const Dev& d, const SimConst& c,
+                                              const SceneGrid& g, const float4& xi, float dx,
+                                              float dy, float dz);
 
 // newtonStepUpdatePosition (src/particles.cpp:206-213) with s_corr (:195)
 // and clamp (:51-84).  in = (x*, lambda), out = (x*', 0).  The last
@@ -500,16 +576,22 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
   visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float w = poly6_sel(c, r2);
-        const float s = c.negk * ipow_n<NT>(w * c.S, c.n_tensile);
-        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
-        const float sc = (xi.w + xj.w) + s;
-        dx = dx + sc * gr.x;
-        dy = dy + sc * gr.y;
-        dz = dz + sc * gr.z;
+        float t[3];
+        deltap_terms<NT>(c, xi, xj, t);
+        dx = dx + t[0];
+        dy = dy + t[1];
+        dz = dz + t[2];
       });
+  deltap_finish<LAST>(i, d, c, g, xi, dx, dy, dz);
+}
+
+// The end of newtonStepUpdatePosition from the summed correction: scale by
+// 1/rho0, clamp (src/particles.cpp:51-84), and store x* (and, in the last
+// iteration, v = (x* - x)/dt).
+template <bool LAST>
+__device__ __forceinline__ void deltap_finish(uint32_t i, const Dev& d, const SimConst& c,
+                                              const SceneGrid& g, const float4& xi, float dx,
+                                              float dy, float dz) {
   dx = dx * c.inv_rho0;
   dy = dy * c.inv_rho0;
   dz = dz * c.inv_rho0;
@@ -517,8 +599,7 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
       clamp_dp(g, c, d.mesh, make_float3(xi.x, xi.y, xi.z), make_float3(dx, dy, dz));
   if (!LAST) {
     // The next λ pass reads x* from the 12-byte records only (this pass
-    // gathers from `in`); `out` is left as it is.
-    (void)out;
+    // gathers from its input x*).
     d.xs3[i] = p;
   } else {
     // The last iteration writes only the (x*, v) record that XSPH gathers.
@@ -529,6 +610,18 @@ __device__ __forceinline__ void deltap_one(uint32_t i, const Dev& d, const SimCo
     r[0] = make_float3(p.x, p.y, p.z);
     r[1] = v;
   }
+}
+
+// The stores of calculateVorticityApplyXSPHViscosity from its sums:
+// out = (x*, |omega|), vxsph = (v + C*visc, density), vort = (omega, 0).
+__device__ __forceinline__ void xsph_finish(uint32_t i, const Dev& d, const SimConst& c,
+                                            const float3& xi, const float3& vi, float ox,
+                                            float oy, float oz, float sx, float sy, float sz,
+                                            float dens, float4* __restrict__ out) {
+  const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
+  out[i] = make_float4(xi.x, xi.y, xi.z, wn);
+  d.vxsph[i] = make_float4(vi.x + c.xc * sx, vi.y + c.xc * sy, vi.z + c.xc * sz, dens);
+  d.vort[i] = make_float4(ox, oy, oz, 0.0f);
 }
 
 // calculateVorticityApplyXSPHViscosity (src/particles.cpp:219-234), Jacobi:
@@ -546,24 +639,22 @@ __device__ __forceinline__ void xsph_one(uint32_t i, const Dev& d, const SimCons
   visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return XV{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
       [&](uint32_t, const XV& q) {
-        const float u0 = q.v.x - vi.x, u1 = q.v.y - vi.y, u2 = q.v.z - vi.z;
-        const float rx = xi.x - q.x.x, ry = xi.y - q.x.y, rz = xi.z - q.x.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
-        ox = ox + (u1 * gr.z - u2 * gr.y);
-        oy = oy + (u2 * gr.x - u0 * gr.z);
-        oz = oz + (u0 * gr.y - u1 * gr.x);
-        const float w = poly6_sel(c, r2);
-        sx = sx + u0 * w;
-        sy = sy + u1 * w;
-        sz = sz + u2 * w;
-        dens = dens + w;
+        float t[7];
+        xsph_terms(c, xi, vi, q.x, q.v, t);
+        ox = ox + t[0];
+        oy = oy + t[1];
+        oz = oz + t[2];
+        sx = sx + t[3];
+        sy = sy + t[4];
+        sz = sz + t[5];
+        dens = dens + t[6];
       });
-  const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
-  out[i] = make_float4(xi.x, xi.y, xi.z, wn);
-  d.vxsph[i] = make_float4(vi.x + c.xc * sx, vi.y + c.xc * sy, vi.z + c.xc * sz, dens);
-  d.vort[i] = make_float4(ox, oy, oz, 0.0f);
+  xsph_finish(i, d, c, xi, vi, ox, oy, oz, sx, sy, sz, dens, out);
 }
+
+__device__ __forceinline__ void vort_finish(uint32_t i, const Dev& d, const SimConst& c,
+                                            const float4& xi, float ex, float ey, float ez,
+                                            float4* x_out, float4* v_out, bool commit);
 
 // applyVorticity + updatePosition (src/particles.cpp:236-248).
 // in = (x*, |omega|).  Commits posA = (x*, id), velA = (v, density).
@@ -578,13 +669,20 @@ __device__ __forceinline__ void vort_one(uint32_t i, const Dev& d, const SimCons
   visit_neighbors<kUnroll>(
       i, cnt, d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj) {
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        const float3 gr = spiky_sel(c, rx, ry, rz, r2);
-        ex = ex + xj.w * gr.x;
-        ey = ey + xj.w * gr.y;
-        ez = ez + xj.w * gr.z;
+        float t[3];
+        vort_terms(c, xi, xj, t);
+        ex = ex + t[0];
+        ey = ey + t[1];
+        ez = ez + t[2];
       });
+  vort_finish(i, d, c, xi, ex, ey, ez, x_out, v_out, commit);
+}
+
+// applyVorticity + updatePosition from eta (src/particles.cpp:238-248):
+// if |eta| > EPS_D, v += dt*eps_v*(eta/|eta| x omega_i); x <- x*.
+__device__ __forceinline__ void vort_finish(uint32_t i, const Dev& d, const SimConst& c,
+                                            const float4& xi, float ex, float ey, float ez,
+                                            float4* x_out, float4* v_out, bool commit) {
   float4 v = d.vxsph[i];
   const float e2 = ex * ex + ey * ey + ez * ez;
   if (sqrtf(e2) > c.eps_d) {
@@ -665,6 +763,21 @@ __global__ __launch_bounds__(kBlock) void k_predict(Dev d, SimConst c, SceneGrid
   const bool valid = i < d.n;
   uint32_t cl = 0xffffffffu;
   if (valid) cl = predict_cell<MOVE>(i, d, c, grid_of<MULTI>(i, g0, d));
+  const uint32_t slot = wave_run_slot(cl, valid, d.cellCount);
+  if (valid) d.arrival[i] = slot;
+}
+
+// The prediction of k_predict<MULTI, true> from the committed state, written
+// to `out` (the array where a prediction fused into the previous step's last
+// pass would be) instead of predP.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_predict_into(Dev d, SimConst c, SceneGrid g0,
+                                                         float4* out) {
+  const uint32_t i = flat_index();
+  const bool valid = i < d.n;
+  uint32_t cl = 0xffffffffu;
+  if (valid)
+    cl = predict_from<true>(i, d, c, grid_of<MULTI>(i, g0, d), d.posA[i], d.velA[i], out);
   const uint32_t slot = wave_run_slot(cl, valid, d.cellCount);
   if (valid) d.arrival[i] = slot;
 }
@@ -852,6 +965,262 @@ __global__ __launch_bounds__(BLOCK) void k_persistent(Dev d, SimConst c, const S
   }
 }
 
+// ------------------------------------------ small states: wide passes --
+// With a few thousand particles a list pass lasts as long as one lane's walk
+// through its own list (about eight dependent chunk gathers for ~31
+// neighbours) while most of the chip idles.  The wide passes give each
+// particle a wave of its own: lane k evaluates the terms of neighbour k
+// with the term functions above, the terms go to LDS, and one lane per sum
+// adds them in list order from +0 -- the additions of the per-lane loops in
+// the same order, hence the same bits.  Lists longer than 64 are taken 64
+// entries at a time, each block added to the running sums before the next.
+constexpr int kWide = 64;            // one wave per particle and workgroup
+constexpr int kWideRow = kWide + 1;  // padded rows: the sum lanes hit distinct banks
+
+// Adds the first m terms of each of the NS series in T to the running sums
+// (lane s < NS holds series s), in order.  Uniform control flow: every lane
+// of the wave calls it.
+template <int NS>
+__device__ __forceinline__ void wide_add(float (*T)[kWideRow], int m, float& acc) {
+  const int lane = (int)threadIdx.x;
+  __syncthreads();
+  if (lane < NS) {
+    int q = 0;
+    for (; q + 4 <= m; q += 4) {
+      const float a0 = T[lane][q], a1 = T[lane][q + 1], a2 = T[lane][q + 2],
+                  a3 = T[lane][q + 3];
+      acc = acc + a0;
+      acc = acc + a1;
+      acc = acc + a2;
+      acc = acc + a3;
+    }
+    for (; q < m; q++) acc = acc + T[lane][q];
+  }
+  __syncthreads();
+}
+
+// The NS sums, on every lane.
+template <int NS>
+__device__ __forceinline__ void wide_result(float (*T)[kWideRow], float acc, float* out) {
+  if ((int)threadIdx.x < NS) T[threadIdx.x][0] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NS; q++) out[q] = T[q][0];
+}
+
+// Sums over the cnt list entries of particle i; term(j, t) gives the NS
+// terms of neighbour j.
+template <int NS, class Term>
+__device__ __forceinline__ void wide_list_sums(uint32_t i, int cnt, const Dev& d, Term term,
+                                               float (*T)[kWideRow], float* out) {
+  const int lane = (int)threadIdx.x;
+  float acc = 0.0f;
+  for (int base = 0; base < cnt; base += kWide) {
+    const int k = base + lane;
+    if (k < cnt) {
+      float t[NS];
+      term(d.nbr[nbr_slot(i, k, d.K4)], t);
+#pragma unroll
+      for (int q = 0; q < NS; q++) T[q][lane] = t[q];
+    }
+    wide_add<NS>(T, min(kWide, cnt - base), acc);
+  }
+  wide_result<NS>(T, acc, out);
+}
+
+// nbr_one and the first lambda of particle blockIdx.x: the lanes test 64
+// candidates at a time, the hits enter the list in candidate order (a ballot
+// and the count of hits in lower lanes give each its slot), and the lambda
+// terms of the hits, whose x* the testing lanes hold, are added block by
+// block.  The same rows, end cells, candidate ranges and test as nbr_one,
+// so the same list; the partial last chunk is padded with its first entry
+// as nbr_one pads it.
+template <bool MULTI>
+__global__ __launch_bounds__(kWide) void k_nbr_lambda_wide(Dev d, SimConst c, SceneGrid g0,
+                                                           float4* out) {
+  __shared__ float T[5][kWideRow];
+  const uint32_t i = blockIdx.x;
+  if (i >= d.n) return;
+  const SceneGrid g = grid_of<MULTI>(i, g0, d);
+  const uint32_t lane = threadIdx.x;
+  const float4 x4 = d.predQ[i];
+  const float3 xi = make_float3(x4.x, x4.y, x4.z);
+  int cc[3];
+  cell_lin(g, xi, cc);
+  const float cell = 1.0f / g.inv_cell;
+  const float h2 = c.h2, hm2 = h2 * 1.0201f;  // (1.01 h)^2
+  const float4* __restrict__ pred = d.predQ;
+  const uint32_t cap = (uint32_t)d.K4 * kChunk;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t cnt = 0;
+  float acc = 0.0f;
+  // The nine (dz, dy) rows, each a contiguous candidate range in sorted
+  // order, found by lanes 0..8 at once (the culling of nbr_one), then walked
+  // as one sequence in row order, 64 candidates per round.
+  __shared__ uint32_t row_s[9], row_at[10];
+  if (lane < 9u) {
+    const int dz = (int)lane / 3 - 1, dy = (int)lane % 3 - 1;
+    const int z = cc[2] + dz, y = cc[1] + dy;
+    uint32_t s = 0, len = 0;
+    if (z >= 0 && z < g.dims[2] && y >= 0 && y < g.dims[1]) {
+      const float gxl = face_gap(xi.x, g.lo[0], cell, cc[0], -1);
+      const float gxr = face_gap(xi.x, g.lo[0], cell, cc[0], 1);
+      const float gz = face_gap(xi.z, g.lo[2], cell, cc[2], dz);
+      const float gy = face_gap(xi.y, g.lo[1], cell, cc[1], dy);
+      const float yz2 = gy * gy + gz * gz;
+      if (!(yz2 > hm2)) {
+        const int x0 = (cc[0] > 0 && gxl * gxl + yz2 <= hm2) ? cc[0] - 1 : cc[0];
+        const int x1 = (cc[0] + 1 < g.dims[0] && gxr * gxr + yz2 <= hm2) ? cc[0] + 1 : cc[0];
+        const uint32_t row = g.cell_base +
+                             ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
+        s = d.cellStart[row + (uint32_t)x0];
+        const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
+        len = e > s ? e - s : 0u;
+      }
+    }
+    row_s[lane] = s;
+    row_at[lane + 1] = len;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    row_at[0] = 0;
+    for (int r = 0; r < 9; r++) row_at[r + 1] += row_at[r];
+  }
+  __syncthreads();
+  const uint32_t total = row_at[9];
+  for (uint32_t tb = 0; tb < total; tb += kWide) {
+    const uint32_t t = tb + lane;
+    bool hit = false;
+    float3 xj = xi;
+    uint32_t j = 0;
+    if (t < total) {
+      int r = 0;
+      while (row_at[r + 1] <= t) r++;
+      j = row_s[r] + (t - row_at[r]);
+      if (j != i) {
+        const float4 q = pred[j];
+        xj = make_float3(q.x, q.y, q.z);
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        hit = rx * rx + ry * ry + rz * rz <= h2;
+      }
+    }
+    const unsigned long long mask = __ballot(hit);
+    const uint32_t before = (uint32_t)__popcll(mask & below);
+    const uint32_t pos = cnt + before;
+    if (hit && pos < cap) {
+      d.nbr[nbr_slot(i, (int)pos, d.K4)] = j;
+      float tm[5];
+      lambda_terms(c, xi, xj, tm);
+#pragma unroll
+      for (int q = 0; q < 5; q++) T[q][before] = tm[q];
+    }
+    const uint32_t m = (uint32_t)__popcll(mask);
+    const uint32_t room = cnt < cap ? cap - cnt : 0u;
+    wide_add<5>(T, (int)min(m, room), acc);
+    cnt += m;
+  }
+  const uint32_t nc = min(cnt, cap);
+  // Pad the partial last chunk (its first entry repeated), as nbr_one does;
+  // the barrier in wide_result orders the list stores before the reads.
+  float sum[5];
+  wide_result<5>(T, acc, sum);
+  if (lane == 0) {
+    const uint32_t r = nc & 3u;
+    if (r) {
+      const int first = (int)(nc - r);
+      const uint32_t e0 = d.nbr[nbr_slot(i, first, d.K4)];
+      for (uint32_t q = r; q < 4u; q++) d.nbr[nbr_slot(i, first + (int)q, d.K4)] = e0;
+    }
+    if (cnt > cap) atomicMax(&d.flags[0], cnt);
+    d.ncnt[i] = nc;
+    out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, sum[0], sum[4], sum[1], sum[2], sum[3]));
+  }
+}
+
+// lambda_one, wide.
+template <bool PACKED>
+__global__ __launch_bounds__(kWide) void k_lambda_wide(Dev d, SimConst c, const float4* in,
+                                                       float4* out) {
+  __shared__ float T[5][kWideRow];
+  const uint32_t i = blockIdx.x;
+  if (i >= d.n) return;
+  const float3 xi = PACKED ? d.xs3[i] : make_float3(in[i].x, in[i].y, in[i].z);
+  float sum[5];
+  wide_list_sums<5>(
+      i, (int)d.ncnt[i], d,
+      [&](uint32_t j, float* t) {
+        float3 xj;
+        if (PACKED) {
+          xj = at(d.xs3, j);
+        } else {
+          const float4 q = at(in, j);
+          xj = make_float3(q.x, q.y, q.z);
+        }
+        lambda_terms(c, xi, xj, t);
+      },
+      T, sum);
+  if (threadIdx.x == 0)
+    out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, sum[0], sum[4], sum[1], sum[2], sum[3]));
+}
+
+// deltap_one, wide.
+template <bool MULTI, bool LAST, int NT>
+__global__ __launch_bounds__(kWide) void k_deltap_wide(Dev d, SimConst c, SceneGrid g0,
+                                                       const float4* in) {
+  __shared__ float T[3][kWideRow];
+  const uint32_t i = blockIdx.x;
+  if (i >= d.n) return;
+  const float4 xi = in[i];
+  float sum[3];
+  wide_list_sums<3>(
+      i, (int)d.ncnt[i], d,
+      [&](uint32_t j, float* t) { deltap_terms<NT>(c, xi, at(in, j), t); }, T, sum);
+  if (threadIdx.x == 0)
+    deltap_finish<LAST>(i, d, c, grid_of<MULTI>(i, g0, d), xi, sum[0], sum[1], sum[2]);
+}
+
+// xsph_one, wide.
+__global__ __launch_bounds__(kWide) void k_xsph_wide(Dev d, SimConst c, float4* out) {
+  __shared__ float T[7][kWideRow];
+  const uint32_t i = blockIdx.x;
+  if (i >= d.n) return;
+  const float3* __restrict__ xv = reinterpret_cast<const float3*>(d.xv);
+  const float3 xi = xv[2 * (size_t)i], vi = xv[2 * (size_t)i + 1];
+  float sum[7];
+  wide_list_sums<7>(
+      i, (int)d.ncnt[i], d,
+      [&](uint32_t j, float* t) { xsph_terms(c, xi, vi, at(xv, 2 * j), at(xv, 2 * j + 1), t); },
+      T, sum);
+  if (threadIdx.x == 0)
+    xsph_finish(i, d, c, xi, vi, sum[0], sum[1], sum[2], sum[3], sum[4], sum[5], sum[6], out);
+}
+
+// vort_one, wide; with PREDICT the next step's prediction, cell and
+// histogram slot follow as in k_vort_commit_predict (one atomic per
+// particle: the slot order inside a cell does not matter, the rank orders
+// by id).
+template <bool PREDICT, bool MULTI>
+__global__ __launch_bounds__(kWide) void k_vort_wide(Dev d, SimConst c, SceneGrid g0,
+                                                     const float4* in) {
+  __shared__ float T[3][kWideRow];
+  const uint32_t i = blockIdx.x;
+  if (i >= d.n) return;
+  const float4 xi = in[i];
+  float sum[3];
+  wide_list_sums<3>(
+      i, (int)d.ncnt[i], d, [&](uint32_t j, float* t) { vort_terms(c, xi, at(in, j), t); }, T,
+      sum);
+  if (threadIdx.x != 0) return;
+  if (!PREDICT) {
+    vort_finish(i, d, c, xi, sum[0], sum[1], sum[2], nullptr, nullptr, true);
+  } else {
+    float4 x, v;
+    vort_finish(i, d, c, xi, sum[0], sum[1], sum[2], &x, &v, false);
+    const uint32_t cl = predict_from<true>(i, d, c, grid_of<MULTI>(i, g0, d), x, v, d.vort);
+    d.arrival[i] = atomicAdd(&d.cellCount[cl], 1u);
+  }
+}
+
 // Scan, scatter and rank of a small state in one workgroup: the three
 // passes that otherwise take four or five launches, with workgroup barriers
 // between them (every thread of the workgroup shares one CU's cache, so a
@@ -866,6 +1235,9 @@ __global__ __launch_bounds__(kSortSmallBlock) void k_sort_small(Dev d, const flo
   for (uint32_t i = threadIdx.x; i < d.n; i += kSortSmallBlock) scatter_one(i, d);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < d.n; i += kSortSmallBlock) rank_one(i, d, pred);
+  // The histogram is spent (the scan above read it): zero it for the next
+  // step's prediction, which then needs no fill launch of its own.
+  for (uint32_t k = threadIdx.x; k < ncell; k += kSortSmallBlock) d.cellCount[k] = 0u;
 }
 
 // ------------------------------------------------------------ read-back --
@@ -1025,8 +1397,18 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
   return hipGetLastError();
 }
 
+static bool wide(const Ctx& x) { return x.n <= x.wide_max; }
+static unsigned wide_grid(const Ctx& x) { return (unsigned)x.n; }
+
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   const Dev d = dev_of(x);
+  if (wide(x)) {
+    if (multi(x))
+      k_nbr_lambda_wide<true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], out);
+    else
+      k_nbr_lambda_wide<false><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], out);
+    return hipGetLastError();
+  }
   if (multi(x))
     k_nbr_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
   else
@@ -1035,6 +1417,13 @@ hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
 }
 
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out, bool packed) {
+  if (wide(x)) {
+    if (packed)
+      k_lambda_wide<true><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, in, out);
+    else
+      k_lambda_wide<false><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, in, out);
+    return hipGetLastError();
+  }
   if (packed)
     k_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
   else
@@ -1046,6 +1435,21 @@ template <int NT>
 static void launch_deltap_nt(const Ctx& x, const Dev& d, const SimConst& c, const float4* in,
                              float4* out, bool last) {
   const unsigned nb = blocks_for(x.n);
+  if (wide(x)) {
+    const unsigned ng = wide_grid(x);
+    if (multi(x)) {
+      if (last)
+        k_deltap_wide<true, true, NT><<<ng, kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+      else
+        k_deltap_wide<true, false, NT><<<ng, kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+    } else {
+      if (last)
+        k_deltap_wide<false, true, NT><<<ng, kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+      else
+        k_deltap_wide<false, false, NT><<<ng, kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+    }
+    return;
+  }
   if (multi(x)) {
     if (last)
       k_deltap<true, true, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
@@ -1069,6 +1473,10 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
 }
 
 hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
+  if (wide(x)) {
+    k_xsph_wide<<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, out);
+    return hipGetLastError();
+  }
   k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, out);
   return hipGetLastError();
 }
@@ -1084,9 +1492,35 @@ hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel) {
 }
 
 hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* in) {
-  hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
-  if (e != hipSuccess) return e;
+  // After the single-workgroup sort the histogram is already zero.
+  const bool zeroed = x.small_sort && sort_in_one_workgroup(x);
   const Dev d = dev_of(x);
+  if (wide(x) && !zeroed) {
+    // Beyond the smallest states the wide pass commits and the prediction
+    // follows as its own launch (the serial tail of one lane per particle,
+    // with one histogram atomic each, costs more than the launch).  It is
+    // written where the fused prediction would be.
+    k_vort_wide<false, false><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream)) != hipSuccess) return e;
+    if (multi(x))
+      k_predict_into<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], d.vort);
+    else
+      k_predict_into<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], d.vort);
+    return hipGetLastError();
+  }
+  if (!zeroed) {
+    const hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
+    if (e != hipSuccess) return e;
+  }
+  if (wide(x)) {
+    if (multi(x))
+      k_vort_wide<true, true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+    else
+      k_vort_wide<true, false><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
+    return hipGetLastError();
+  }
   if (multi(x))
     k_vort_commit_predict<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
   else
@@ -1095,6 +1529,10 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
 }
 
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
+  if (wide(x)) {
+    k_vort_wide<false, false><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, x.grids[0], in);
+    return hipGetLastError();
+  }
   k_vort_commit<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in);
   return hipGetLastError();
 }

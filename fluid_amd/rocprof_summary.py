@@ -194,7 +194,8 @@ def counters(args) -> None:
             base = re.sub(r"<.*", "", k)
             fam[VARIANT_OF.get(base, base)].append((n, rec))
         tot = {}
-        for f_ in ("hbm_read_bytes", "hbm_write_bytes", "valu_insts"):
+        for f_ in ("hbm_read_bytes", "hbm_write_bytes", "valu_insts", "lds_bank_conflict",
+                   "lds_idx_active"):
             tot[f_] = sum(sum(n * r[f_] for n, r in v) / sum(n for n, _ in v) for v in fam.values())
         tot["hbm_bytes"] = tot["hbm_read_bytes"] + tot["hbm_write_bytes"]
         tas = [(n, r["ta_busy_frac"]) for _, n, r in recs if "ta_busy_frac" in r]

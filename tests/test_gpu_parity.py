@@ -224,6 +224,41 @@ def test_small_state_single_workgroup_sort_over_several_scan_chunks(orc):
         assert bits_equal(gx, xg) and bits_equal(gv, vg)
 
 
+@pytest.mark.parametrize("spacing_over_h", [0.5, 0.3])
+def test_wide_passes_equal_per_lane_passes(orc, spacing_over_h):
+    """States of up to 4,096 particles run the list passes with one wave per
+    particle; contexts created with PBF_WIDE_MAX=0 run the per-lane passes.
+    Same bits for the dam block (about 31 neighbours) and for a block at
+    spacing 0.3 h (about 150 neighbours at first, 98 at most after three
+    steps: lists of several 64-entry blocks),
+    and both equal the oracle."""
+    import os
+
+    sc = scenes.dam_block()
+    h = sc.params.h
+    s = spacing_over_h * h
+    x = scenes.lattice_block(16, 16, 16, s, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += np.random.default_rng(4).uniform(-1e-3 * s, 1e-3 * s, x.shape).astype(np.float32)
+    v = np.random.default_rng(5).uniform(-0.2, 0.2, x.shape).astype(np.float32)
+    p = sc.params.replace(max_neighbors=512)
+    xg, vg, rg = orc.step(p, x, v, 3, orc.NBR_GRID)
+    outs = []
+    for env in (None, "0"):
+        if env is not None:
+            os.environ["PBF_WIDE_MAX"] = env
+        try:
+            with ctx_for(p, x, v) as c:
+                c.time_step(nsteps=3)
+                outs.append(c.read())
+                nbar, nmax = c.neighbor_stats()
+        finally:
+            os.environ.pop("PBF_WIDE_MAX", None)
+    if spacing_over_h < 0.4:
+        assert nmax > 64  # several 64-entry blocks per list
+    for gx, gv, gr in outs:
+        assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
+
+
 def test_two_sphere_collision_bit_exact(orc):
     """A small two-sphere impact (v_x = +-2) with the c2 constants."""
     sc = scenes.two_spheres(per_sphere=3000, spacing=0.04, steps=5)
