@@ -28,6 +28,8 @@ HBM.  Prints ONE JSON line:
 * passes     ms per step, GB/s (state-only and state+list) of every pass,
              and the LDS bank-conflict rate (SQ_LDS_BANK_CONFLICT /
              SQ_LDS_IDX_ACTIVE) from the committed counters;
+* dam_break  the 262,144-particle dam break's whole 200-step run, in windows
+             of 50 steps, with the same per-pass table;
 * small_scenes  the reference's own scenes (particles/p.xml, 720 particles,
              and particles/spheres_p.xml, 2,106) at its native constants;
 * reference_distance  per-step error and 100-step drift against the FP64
@@ -289,6 +291,99 @@ def mesh_bench(sc, warmup: int, steps: int, plain_ms: float):
     }
 
 
+def pass_table(prof, sc, steps: int, nbar: float, counters) -> dict:
+    """Per pass: ms per step, launches, mean launch time and state-only and
+    state+list GB/s from HIP-event times `prof` ({pass: (ms, launches)} over
+    `steps` steps); HBM GB/s, VALU fraction, LDS bank-conflict rate and TA
+    busy from the committed rocprofv3 counters of the workload, if any."""
+    ncell = 1
+    for a in range(3):
+        ncell *= int((sc.params.box_hi[a] - sc.params.box_lo[a]) / (sc.params.h * sc.params.cell_scale)) + 1
+    passes = {}
+    for name, (ms, launches) in prof.items():
+        if launches == 0:
+            continue
+        per_launch_s = ms / launches * 1e-3
+        state = STATE_BYTES[name] * sc.n + (8 * ncell if name == "scan" else 0)
+        listb = (4.0 * nbar + 4.0) * sc.n if name in LIST_PASSES else 0.0
+        pd = {
+            "ms_per_step": ms / steps,
+            "launches_per_step": launches / steps,
+            "us_per_launch": per_launch_s * 1e6,
+            "state_GBs": state / per_launch_s / 1e9,
+            "state_list_GBs": (state + listb) / per_launch_s / 1e9,
+        }
+        c = counters.get(name, {})
+        if c.get("hbm_bytes"):
+            pd["hbm_GBs"] = c["hbm_bytes"] / per_launch_s / 1e9
+        if c.get("valu_insts"):
+            pd["valu_frac"] = c["valu_insts"] * 64 / per_launch_s / VALU_PEAK_LANE_OPS
+        if c.get("lds_idx_active"):
+            pd["lds_bank_conflict_rate"] = c.get("lds_bank_conflict", 0.0) / c["lds_idx_active"]
+        elif "lds_idx_active" in c:
+            pd["lds_bank_conflict_rate"] = None  # the pass uses no LDS
+        if c.get("ta_busy_frac") is not None:
+            # Share of the launch with gathers in flight in each CU's
+            # address unit (TA_TA_BUSY_sum / CUs / GRBM_GUI_ACTIVE per XCD).
+            pd["ta_busy_frac"] = c["ta_busy_frac"]
+        passes[name] = pd
+    return passes
+
+
+def dam_break_bench(warmup: int):
+    """The 262,144-particle dam break (64^3 lattice at spacing 0.025, h 0.05,
+    4 iterations, dt 10 ms): its whole 200-step run from the initial state,
+    then the same steps with HIP events around every launch, in windows of
+    50 steps, with the per-pass table of the headline scene."""
+    from fluid_amd import scenes
+    from fluid_amd.pbf import Particles
+
+    sc = scenes.dam_break()
+    steps = sc.steps
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.time_step(nsteps=warmup)
+        c.load(sc.x, sc.v)
+        c.sync()
+        t0 = time.perf_counter()
+        c.time_step(nsteps=steps)
+        c.sync()
+        wall = time.perf_counter() - t0
+        c.load(sc.x, sc.v)
+        prof, windows = {}, []
+        for a in range(0, steps, 50):
+            t0 = time.perf_counter()
+            pw = c.time_step_profiled(nsteps=50)  # returns after the device finished
+            ww = time.perf_counter() - t0
+            wbar, wmax = c.neighbor_stats()
+            windows.append({
+                "steps": f"{a + 1}-{a + 50}",
+                "ms_per_step": ww / 50 * 1e3,
+                "pass_ms_per_step": {k: ms / 50 for k, (ms, n) in pw.items() if n},
+                "mean_neighbors_last_step": wbar,
+                "max_neighbors_last_step": wmax,
+            })
+            for k, (ms, n) in pw.items():
+                m0, n0 = prof.get(k, (0.0, 0))
+                prof[k] = (m0 + ms, n0 + n)
+        nbar, nmax = c.neighbor_stats()
+    counters = (load_counters(sc.name) or {}).get("passes", {})
+    res = {
+        "workload": sc.name,
+        "particles": sc.n,
+        "steps": steps,
+        "ms_per_step": wall / steps * 1e3,
+        "particle_steps_per_s": sc.n * steps / wall,
+        "mean_neighbors": nbar,
+        "max_neighbors": nmax,
+        "windows": windows,
+        "passes": pass_table(prof, sc, steps, nbar, counters),
+    }
+    log(f"dam break {sc.n}: {steps} steps at {res['ms_per_step']:.3f} ms/step, neighbours "
+        f"mean {nbar:.1f} max {nmax} at the end")
+    return res
+
+
 def small_scenes_bench(warmup: int = 10, steps: int = 300):
     """The reference's own scenes, particles/p.xml (720 particles) and
     particles/spheres_p.xml (2,106), loaded as its -p path does, at its
@@ -370,6 +465,9 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--no-dam-break", action="store_true")
+    ap.add_argument("--only-dam-break", action="store_true",
+                    help="run only the 262,144-particle dam break and print its result")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--looped-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
@@ -389,6 +487,10 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: 3 x 256 MiB read + 256 MiB write")
+
+    if args.only_dam_break:
+        print(json.dumps({"dam_break": dam_break_bench(args.warmup)}), flush=True)
+        return 0
 
     from fluid_amd import scenes
     from fluid_amd.pbf import Particles
@@ -457,28 +559,11 @@ def main() -> int:
     field = density_field_bench(ctx, sc)
     snap = snapshot_bench(ctx, sc)
     ctx.close()
-    ps = args.steps
-    ncell = 1
-    for a in range(3):
-        ncell *= int((sc.params.box_hi[a] - sc.params.box_lo[a]) / (sc.params.h * sc.params.cell_scale)) + 1
-    passes = {}
-    for name, (ms, launches) in prof.items():
-        if launches == 0:
-            continue
-        per_launch_s = ms / launches * 1e-3
-        state = STATE_BYTES[name] * sc.n + (8 * ncell if name == "scan" else 0)
-        listb = (4.0 * nbar + 4.0) * sc.n if name in LIST_PASSES else 0.0
-        passes[name] = {
-            "ms_per_step": ms / ps,
-            "launches_per_step": launches / ps,
-            "us_per_launch": per_launch_s * 1e6,
-            "state_GBs": state / per_launch_s / 1e9,
-            "state_list_GBs": (state + listb) / per_launch_s / 1e9,
-        }
+    counters = (load_counters(sc.name) or {}).get("passes", {})
+    passes = pass_table(prof, sc, args.steps, nbar, counters)
     dom = max(passes, key=lambda k: passes[k]["ms_per_step"])
     d = passes[dom]
     per_launch_bytes = STATE_BYTES[dom] * sc.n
-    counters = (load_counters(sc.name) or {}).get("passes", {})
     cp = counters.get(dom, {})
     roof = {
         # The passes are bound by VALU issue (and their neighbour gathers),
@@ -505,20 +590,6 @@ def main() -> int:
             "frac": lane_ops / VALU_PEAK_LANE_OPS,
             "valu_insts_per_launch": cp["valu_insts"],
         }
-    for name, pd in passes.items():
-        c = counters.get(name, {})
-        if c.get("hbm_bytes"):
-            pd["hbm_GBs"] = c["hbm_bytes"] / (pd["us_per_launch"] * 1e-6) / 1e9
-        if c.get("valu_insts"):
-            pd["valu_frac"] = c["valu_insts"] * 64 / (pd["us_per_launch"] * 1e-6) / VALU_PEAK_LANE_OPS
-        if c.get("lds_idx_active"):
-            pd["lds_bank_conflict_rate"] = c.get("lds_bank_conflict", 0.0) / c["lds_idx_active"]
-        elif "lds_idx_active" in c:
-            pd["lds_bank_conflict_rate"] = None  # the pass uses no LDS
-        if c.get("ta_busy_frac") is not None:
-            # Share of the launch with gathers in flight in each CU's
-            # address unit (TA_TA_BUSY_sum / CUs / GRBM_GUI_ACTIVE per XCD).
-            pd["ta_busy_frac"] = c["ta_busy_frac"]
     out = {
         "metric": "PBF particle-steps/s @4M",
         "value": value,
@@ -558,6 +629,8 @@ def main() -> int:
             out["reference_distance"] = json.load(f)
     if not args.no_mesh:
         out["mesh"] = mesh_bench(sc, args.warmup, 10, wall / args.steps * 1e3)
+    if not args.no_dam_break:
+        out["dam_break"] = dam_break_bench(args.warmup)
     if not args.no_small:
         out["small_scenes"] = small_scenes_bench()
     if not args.no_batched:

@@ -2,7 +2,8 @@
 collision, the 262,144-particle dam break and the 512-scene batch, against
 the oracle (bit-exact, grid/Jacobi) and through size-independent properties
 (determinism across runs and schedules, the batch equal to the plain path
-scene by scene)."""
+scene by scene over 10 steps: the batch runs the per-lane passes, the
+scenes alone the wave-per-particle passes of small states)."""
 import numpy as np
 import pytest
 
@@ -75,14 +76,26 @@ def test_two_spheres_4m_deterministic_across_runs_and_schedules(two_spheres):
 
 
 def test_dam_break_262k_steps_bit_exact(orc):
+    """20 of the dam break's 200 steps, bit-exact, then its last 5 from the
+    state after step 195 (read back from the GPU), when the column has
+    spread along the floor and compressed (mean ~96 neighbours, lists of up
+    to ~900)."""
     sc = scenes.dam_break()
     assert sc.n == 262_144
     with Particles(sc.params) as c:
         c.load(sc.x, sc.v)
-        c.time_step(nsteps=3)
+        c.time_step(nsteps=20)
         gx, gv, _ = c.read()
-    ox, ov, _ = orc.step(sc.params, sc.x, sc.v, 3, orc.NBR_GRID)
+        c.time_step(nsteps=175)
+        x195, v195, _ = c.read()
+        c.time_step(nsteps=5)
+        hx, hv, _ = c.read()
+        nbar, nmax = c.neighbor_stats()
+    assert nbar > 80 and nmax > 512
+    ox, ov, _ = orc.step(sc.params, sc.x, sc.v, 20, orc.NBR_GRID)
     assert bits_equal(gx, ox) and bits_equal(gv, ov)
+    ox, ov, _ = orc.step(sc.params, x195, v195, 5, orc.NBR_GRID)
+    assert bits_equal(hx, ox) and bits_equal(hv, ov)
 
 
 def test_batch_512_scenes_equal_plain_path():
@@ -91,13 +104,13 @@ def test_batch_512_scenes_equal_plain_path():
     spec = [(len(x), lo, hi) for x, _, lo, hi in items]
     with Particles(p, scenes=spec) as b:
         b.load(np.concatenate([it[0] for it in items]), np.concatenate([it[1] for it in items]))
-        b.time_step(nsteps=2)
+        b.time_step(nsteps=10)
         bx, bv, _ = b.read()
     off = 0
     for x, v, lo, hi in items:
         with Particles(p.replace(box_lo=lo, box_hi=hi)) as c:
             c.load(x, v)
-            c.time_step(nsteps=2)
+            c.time_step(nsteps=10)
             gx, gv, _ = c.read()
         n = len(x)
         assert bits_equal(bx[off:off + n], gx) and bits_equal(bv[off:off + n], gv)

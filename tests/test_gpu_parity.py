@@ -226,7 +226,7 @@ def test_small_state_single_workgroup_sort_over_several_scan_chunks(orc):
 
 @pytest.mark.parametrize("spacing_over_h", [0.5, 0.3])
 def test_wide_passes_equal_per_lane_passes(orc, spacing_over_h):
-    """States of up to 4,096 particles run the list passes with one wave per
+    """States of up to 32,768 particles run the list passes with one wave per
     particle; contexts created with PBF_WIDE_MAX=0 run the per-lane passes.
     Same bits for the dam block (about 31 neighbours) and for a block at
     spacing 0.3 h (about 150 neighbours at first, 98 at most after three
@@ -243,16 +243,16 @@ def test_wide_passes_equal_per_lane_passes(orc, spacing_over_h):
     p = sc.params.replace(max_neighbors=512)
     xg, vg, rg = orc.step(p, x, v, 3, orc.NBR_GRID)
     outs = []
-    for env in (None, "0"):
-        if env is not None:
-            os.environ["PBF_WIDE_MAX"] = env
+    for env in ({}, {"PBF_WIDE_MAX": "0"}):
+        os.environ.update(env)
         try:
             with ctx_for(p, x, v) as c:
                 c.time_step(nsteps=3)
                 outs.append(c.read())
                 nbar, nmax = c.neighbor_stats()
         finally:
-            os.environ.pop("PBF_WIDE_MAX", None)
+            for k in env:
+                os.environ.pop(k, None)
     if spacing_over_h < 0.4:
         assert nmax > 64  # several 64-entry blocks per list
     for gx, gv, gr in outs:
