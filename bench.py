@@ -468,6 +468,10 @@ def main() -> int:
     ap.add_argument("--no-dam-break", action="store_true")
     ap.add_argument("--only-dam-break", action="store_true",
                     help="run only the 262,144-particle dam break and print its result")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the 512-scene batch against the looped plain path")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the 32,768-particle, 60-step adjoint rollout")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--looped-steps", type=int, default=10)
     ap.add_argument("--calibrate", action="store_true",
@@ -490,6 +494,13 @@ def main() -> int:
 
     if args.only_dam_break:
         print(json.dumps({"dam_break": dam_break_bench(args.warmup)}), flush=True)
+        return 0
+    if args.only_batched:
+        print(json.dumps({"batched": batched_bench(args.batched_steps, args.looped_steps)}),
+              flush=True)
+        return 0
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_bench()}), flush=True)
         return 0
 
     from fluid_amd import scenes
