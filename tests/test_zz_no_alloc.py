@@ -87,6 +87,27 @@ def test_noalloc_matches_allocating_path_on_two_calls():
             assert not same_bits(gx, np.asarray(x_in, np.float32))
 
 
+def test_noalloc_lane_per_particle_state():
+    """A 40,000-particle block (above the wave-per-particle range, so the
+    lane-per-particle passes run; within the 64 MiB workspace at 256
+    neighbours), then every second particle (20,000: wave-per-particle
+    passes) in the same workspace: the allocating path's bits both times."""
+    from fluid_amd import params as P
+
+    p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
+    rng = np.random.default_rng(11)
+    x = scenes.lattice_block(40, 40, 25, 0.05, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += rng.uniform(-1e-3, 1e-3, x.shape).astype(np.float32)
+    v = rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)
+    assert len(x) == 40_000
+    with Particles(p) as ctx:
+        for x_in, v_in, steps in ((x, v, 3), (x[::2], v[::2], 2)):
+            print(f"noalloc: {len(x_in)} particles, {steps} steps", flush=True)
+            gx, gv = noalloc_path(ctx, x_in, v_in, steps, p.dt)
+            rx, rv = allocating_path(p, x_in, v_in, steps)
+            assert same_bits(gx, rx) and same_bits(gv, rv)
+
+
 def test_noalloc_refuses_sizes_beyond_the_workspace_and_bad_input():
     sc = scenes.dam_block()
     with Particles(sc.params) as ctx:
