@@ -75,6 +75,34 @@ def test_falling_block_on_sphere_bit_exact(orc_mesh):
     assert r.min() > 0.18 and (r < 0.25).sum() > 50
 
 
+def test_falling_slab_on_sphere_lane_per_particle_bit_exact(orc_mesh):
+    """A 48x20x48 slab (46,080 particles, above the wave-per-particle range,
+    so the lane-per-particle passes and their collision calls run) falling
+    onto the sphere: the oracle's bits after 15 and 30 steps."""
+    orc = orc_mesh
+    rng = np.random.default_rng(5)
+    s = 0.02
+    x = scenes.lattice_block(48, 20, 48, s, (0.03, 0.54, 0.03))
+    x = (x + rng.uniform(-5e-4, 5e-4, x.shape)).astype(np.float32)
+    assert len(x) == 46_080 and x.max() < 1.0
+    p = P.scene_params(2 * s, s, (0, 0, 0), (1, 1, 1), dt=0.01)
+    m = mesh.merge(mesh.box((0, 0, 0), (1, 1, 1)), mesh.icosphere((0.5, 0.3, 0.5), 0.2, 2))
+    m = m + (mesh.default_skin(m[0], (0, 0, 0), (1, 1, 1)),)
+    orc.set_mesh(*m)
+    v = np.zeros_like(x)
+    with Particles(p) as c:
+        c.set_mesh(*m)
+        c.load(x, v)
+        ox, ov = x, v
+        for _ in range(2):
+            c.time_step(nsteps=15)
+            ox, ov, _ = orc.step(p, ox, ov, 15, orc.NBR_GRID)
+            gx, gv, _ = c.read()
+            assert bits_equal(gx, ox) and bits_equal(gv, ov)
+    r = np.linalg.norm(gx - np.array([0.5, 0.3, 0.5]), axis=1)
+    assert r.min() > 0.18 and (r < 0.25).sum() > 100  # on the sphere, none inside it
+
+
 def test_box_mesh_matches_box_walls_and_removal_restores_them():
     sc = scenes.dam_block()
     p = sc.params
