@@ -43,6 +43,33 @@ def test_resume_is_bit_exact(tmp_path):
             assert bits_equal(a, b)
 
 
+def test_resume_lane_per_particle_state_is_bit_exact(tmp_path):
+    """40,000 particles (lane-per-particle passes, whose last pass of a step
+    also predicts the next step when the call goes on): 12 steps in one call
+    against 6, a snapshot, a fresh context and 6 more."""
+    from fluid_amd import params as P
+
+    p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
+    rng = np.random.default_rng(13)
+    x = scenes.lattice_block(40, 40, 25, 0.05, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += rng.uniform(-1e-3, 1e-3, x.shape).astype(np.float32)
+    v = rng.uniform(-0.5, 0.5, x.shape).astype(np.float32)
+    path = str(tmp_path / "lane.pbfsnap")
+    with Particles(p) as c:
+        c.load(x, v)
+        c.time_step(nsteps=12)
+        whole = c.read()
+    with Particles(p) as c:
+        c.load(x, v)
+        c.time_step(nsteps=6)
+        c.save_snapshot(path)
+    with Particles(p) as r:
+        r.load_snapshot(path)
+        r.time_step(nsteps=6)
+        for a, b in zip(r.read(), whole):
+            assert bits_equal(a, b)
+
+
 def test_batched_resume_is_bit_exact(tmp_path):
     items, p = scenes.batched(nscenes=5, nmin=500, nmax=2000)
     spec = [(len(x), lo, hi) for x, _, lo, hi in items]
