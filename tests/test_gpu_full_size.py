@@ -98,6 +98,27 @@ def test_dam_break_262k_steps_bit_exact(orc):
     assert bits_equal(hx, ox) and bits_equal(hv, ov)
 
 
+def test_lane_per_particle_100_steps_bit_exact(orc):
+    """100 steps of a 40,000-particle block with random velocities (the
+    lane-per-particle passes; walls reached on every side) in one call:
+    the oracle's bits, so the 100-step drift of this path is 0 too."""
+    from fluid_amd import params as P
+
+    p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
+    rng = np.random.default_rng(21)
+    x = scenes.lattice_block(40, 40, 25, 0.05, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += rng.uniform(-1e-3, 1e-3, x.shape).astype(np.float32)
+    v = rng.uniform(-1.0, 1.0, x.shape).astype(np.float32)
+    with Particles(p) as c:
+        c.load(x, v)
+        c.time_step(nsteps=100)
+        gx, gv, gr = c.read()
+    ox, ov, orho = orc.step(p, x, v, 100, orc.NBR_GRID)
+    assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(gr, orho)
+    lo = p.box_lo[0] + 0.05
+    assert (gx.min(axis=0) < lo).all()  # particles at the walls
+
+
 def test_batch_512_scenes_equal_plain_path():
     items, p = scenes.batched()
     assert len(items) == 512
