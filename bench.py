@@ -41,7 +41,10 @@ HBM.  Prints ONE JSON line:
              lattice, checkpoint files, the triangle-mesh collider, many
              small scenes, gradients);
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
-             every host core) timed on a few steps of the same scene.
+             every host core) timed on a few steps of the same scene;
+* cpu_workloads  the same restatement on the dam block's 100 steps and
+             steps 1-50 of the dam break (all cores; one core on 100 and 2
+             steps), beside the GPU's rate on those workloads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -128,6 +131,56 @@ def cpu_baseline(sc, steps: int):
         "one_core": {"value": sc.n / dt1, "unit": "particle-steps/s", "cores": 1,
                      "sample": f"{sc.name}: 1 step, {dt1:.2f} s"},
     }
+
+
+def cpu_workloads(dam_break_gpu):
+    """The CPU restatement on the dam block (4,096 particles, its 100 steps,
+    all host cores and one) and on a bounded sample of the dam break
+    (262,144 particles: steps 1-50 of its 200 on all cores, ~10 s; steps 1-2
+    on one), beside the GPU's rate (the dam block's 100 steps timed here,
+    the dam break's whole run from its bench entry, whose later steps carry
+    three times the neighbours of the sample's)."""
+    from fluid_amd import scenes
+    from fluid_amd.pbf import Particles
+    from oracle import oracle
+
+    threads = len(os.sched_getaffinity(0))
+    out = {}
+
+    def cpu(sc, steps, nthreads):
+        oracle.set_threads(nthreads)
+        t0 = time.perf_counter()
+        oracle.step(sc.params, sc.x, sc.v, steps)
+        dt = time.perf_counter() - t0
+        return {"value": sc.n * steps / dt, "unit": "particle-steps/s", "cores": nthreads,
+                "steps": steps, "seconds": dt}
+
+    sc = scenes.dam_block()
+    with Particles(sc.params) as c:
+        c.load(sc.x, sc.v)
+        c.time_step(nsteps=sc.steps)
+        c.load(sc.x, sc.v)
+        c.sync()
+        t0 = time.perf_counter()
+        c.time_step(nsteps=sc.steps)
+        c.sync()
+        wall = time.perf_counter() - t0
+    out[sc.name] = {"gpu": {"value": sc.n * sc.steps / wall, "unit": "particle-steps/s",
+                            "steps": sc.steps, "ms_per_step": wall / sc.steps * 1e3},
+                    "cpu_all_cores": cpu(sc, sc.steps, threads),
+                    "cpu_one_core": cpu(sc, sc.steps, 1)}
+    sc = scenes.dam_break()
+    entry = {"cpu_all_cores": cpu(sc, 50, threads), "cpu_one_core": cpu(sc, 2, 1)}
+    if dam_break_gpu:
+        entry["gpu"] = {"value": dam_break_gpu["particle_steps_per_s"], "unit": "particle-steps/s",
+                        "steps": dam_break_gpu["steps"], "ms_per_step": dam_break_gpu["ms_per_step"]}
+    out[sc.name] = entry
+    oracle.set_threads(threads)
+    for name, e in out.items():
+        log(f"cpu {name}: {e['cpu_all_cores']['value']:.3e} particle-steps/s on {threads} cores, "
+            f"{e['cpu_one_core']['value']:.3e} on 1" +
+            (f"; GPU {e['gpu']['value']:.3e}" if "gpu" in e else ""))
+    return out
 
 
 def batched_bench(steps: int, looped_steps: int):
@@ -650,6 +703,7 @@ def main() -> int:
         out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
+        out["cpu_workloads"] = cpu_workloads(out.get("dam_break"))
     print(json.dumps(out), flush=True)
     return 0
 
