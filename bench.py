@@ -41,10 +41,10 @@ HBM.  Prints ONE JSON line:
              lattice, checkpoint files, the triangle-mesh collider, many
              small scenes, gradients);
 * cpu_baseline  the CPU restatement (oracle/, FP32, grid, Jacobi, OpenMP on
-             every host core) timed on a few steps of the same scene;
-* cpu_workloads  the same restatement on the dam block's 100 steps and
-             steps 1-50 of the dam break (all cores; one core on 100 and 2
-             steps), beside the GPU's rate on those workloads.
+             every host core) timed on a few steps of the same scene, and
+             (`workloads`) on the dam block's 100 steps and steps 1-50 of
+             the dam break (all cores; one core on 100 and 2 steps), beside
+             the GPU's rate on those workloads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -133,7 +133,7 @@ def cpu_baseline(sc, steps: int):
     }
 
 
-def cpu_workloads(dam_break_gpu):
+def cpu_baseline_workloads(dam_break_gpu):
     """The CPU restatement on the dam block (4,096 particles, its 100 steps,
     all host cores and one) and on a bounded sample of the dam break
     (262,144 particles: steps 1-50 of its 200 on all cores, ~10 s; steps 1-2
@@ -703,7 +703,7 @@ def main() -> int:
         out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps)
-        out["cpu_workloads"] = cpu_workloads(out.get("dam_break"))
+        out["cpu_baseline"]["workloads"] = cpu_baseline_workloads(out.get("dam_break"))
     print(json.dumps(out), flush=True)
     return 0
 
