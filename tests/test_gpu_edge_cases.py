@@ -194,3 +194,55 @@ def test_failed_allocation_leaves_the_context_unloaded():
         c.time_step(nsteps=1)
         x, _, _ = c.read()
         assert x.shape == (500, 3) and np.isfinite(x).all()
+
+
+def test_contexts_on_concurrent_host_threads():
+    """Contexts share no device state: two of them (a 4,096-particle block on
+    the wave-per-particle passes and a 40,000-particle block on the
+    lane-per-particle passes) stepped at the same time from two host
+    threads, each 20 calls of 2 steps with a read back after each, give the
+    bits of the same calls made one context after the other.  An error on
+    one thread (a bad load) does not show up in the other's last error."""
+    import threading
+
+    from fluid_amd import params as P
+
+    small = scenes.dam_block()
+    p_big = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
+    rng = np.random.default_rng(31)
+    xb = scenes.lattice_block(40, 40, 25, 0.05, (0.1, 0.1, 0.1)).astype(np.float32)
+    xb += rng.uniform(-1e-3, 1e-3, xb.shape).astype(np.float32)
+    vb = rng.uniform(-0.5, 0.5, xb.shape).astype(np.float32)
+    jobs = [(small.params, small.x, small.v), (p_big, xb, vb)]
+
+    def work(p, x, v, out, idx, bad):
+        with Particles(p) as c:
+            c.load(x, v)
+            states = []
+            for k in range(20):
+                c.time_step(nsteps=2)
+                states.append(c.read())
+                if bad and k == 10:
+                    with pytest.raises(PbfError):
+                        c.load(np.full((4, 3), np.nan, np.float32), None)
+                    c.load(*states[-1][:2])  # back to the state just read
+            out[idx] = states
+
+    seq = [None, None]
+    for i, (p, x, v) in enumerate(jobs):
+        work(p, x, v, seq, i, False)
+    par = [None, None]
+    threads = [threading.Thread(target=work, args=(p, x, v, par, i, i == 0))
+               for i, (p, x, v) in enumerate(jobs)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for i in range(2):
+        assert par[i] is not None
+        for k in range(10):  # before the reload on thread 0 (bits of every call)
+            for a, b in zip(par[i][k], seq[i][k]):
+                assert bits_equal(a, b), (i, k)
+    for k in range(10, 20):  # thread 1 never saw thread 0's error
+        for a, b in zip(par[1][k], seq[1][k]):
+            assert bits_equal(a, b), (1, k)
