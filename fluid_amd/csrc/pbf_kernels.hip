@@ -179,14 +179,13 @@ __device__ __forceinline__ void store_agent(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier between two passes of the persistent kernel: every write of the
-// pass is released to L2 before the barrier, and the per-CU cache is
-// invalidated after it, so the next pass reads what the other waves wrote.
-__device__ __forceinline__ void phase_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
+// Barrier between two passes of the persistent kernel.  A scene belongs
+// to one workgroup for all its steps and no other workgroup reads its
+// arrays, so a workgroup barrier (which makes the workgroup's global writes
+// visible to all its threads; its waves share one CU) is enough.  The
+// histogram, which the prediction fills with atomics performed in L2, is
+// zeroed and read with agent-scope accesses (store_agent, load_agent).
+__device__ __forceinline__ void phase_barrier() { __syncthreads(); }
 
 // Exclusive scan of in[0..n) into out[0..n) inside one workgroup of BLOCK
 // threads, every value offset by base; out[n] = base + total.  All threads
