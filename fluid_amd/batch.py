@@ -95,6 +95,9 @@ def candidate_plans(counts) -> list:
         if (counts <= thr).any() and (counts > thr).any():
             for b in (256, 512):
                 plans.append(Plan((Group("persistent", b, thr), Group("graph"))))
+                plans.append(Plan((Group("persistent", b, thr), Group("flat"))))
+            # small and large scenes as two flat contexts on concurrent streams
+            plans.append(Plan((Group("flat", 256, thr), Group("flat"))))
     return plans
 
 
