@@ -150,6 +150,11 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
                                    float3* n_out, float3* nf_out) {
   float bt = tmax;
   int best = -1;
+  // The slab tests' reciprocals of the direction, once per query.
+  const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
+  float inva[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) inva[a] = da[a] == 0.0f ? 0.0f : 1.0f / da[a];
   // Stackless walk: a missed box or a finished leaf continues at the escape
   // node, a hit inner box at its left child (the next node).  Every move
   // goes to a larger index, so the walk ends within nnode nodes.
@@ -161,14 +166,13 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
     // needs the origin inside the slab.
     float t0 = 0.0f, t1 = bt;
     bool miss = false;
-    const float oa[3] = {o.x, o.y, o.z}, da[3] = {d.x, d.y, d.z};
     const float la[3] = {lo.x, lo.y, lo.z}, ha[3] = {hi.x, hi.y, hi.z};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
       if (da[a] == 0.0f) {
         miss = miss || oa[a] < la[a] || oa[a] > ha[a];
       } else {
-        const float inv = 1.0f / da[a];
+        const float inv = inva[a];
         float ta = (la[a] - oa[a]) * inv, tb = (ha[a] - oa[a]) * inv;
         if (ta > tb) {
           const float tt = ta;
@@ -194,7 +198,17 @@ __device__ inline int mesh_nearest(const MeshView& m, float3 o, float3 d, float 
         const float3 s1 = cross3(d, e2), s2 = cross3(sv, e1);
         const float det = dot3(s1, e1);
         if (det == 0.0f) continue;
-        const float t = dot3(s2, e2) / det;
+        const float num = dot3(s2, e2);
+        // Rejections the division below would make anyway, without it:
+        // t < 0 (num and det of opposite signs; a zero or NaN num is not
+        // taken), and t > bt where num/det exceeds bt by more than the
+        // roundings of the two products below and of the quotient (1e-6 >
+        // 16 ulp; both products normal, so each is within half an ulp), so
+        // t = RN(num/det) > bt for certain.
+        if ((num < 0.0f && det > 0.0f) || (num > 0.0f && det < 0.0f)) continue;
+        const float btm = bt * 1.000001f, lim = fabsf(det) * btm;
+        if (btm >= 0x1p-126f && lim >= 0x1p-126f && fabsf(num) > lim) continue;
+        const float t = num / det;
         const int idx = (int)__float_as_uint(P.w);
         if (t < 0.0f || t > bt || (t == bt && best >= 0 && idx > best)) continue;
         const float u = dot3(s1, sv) / det;

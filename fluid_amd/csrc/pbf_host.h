@@ -306,8 +306,14 @@ inline std::string read_snapshot(const char* path, const SnapBody& mine, SnapHea
 }
 
 // ---- mesh collider: BVH build ------------------------------------------------
-// Median split on the longest axis of the centroid bounds (ties by index, so
-// the tree is deterministic), leaves of at most 4 triangles.  Nodes are
+// At each node, triangles whose box diagonal exceeds half the node box's go
+// to a subtree of their own (the first child, index order kept), so a few
+// large faces (the walls of a box around a finely meshed obstacle) do not
+// widen every box on the way to the small ones: a walk near such an
+// obstacle then visits ~32 nodes instead of ~125.  Otherwise (none or all
+// of them large), a median split on the longest axis of the centroid
+// bounds (ties by index, so the tree is deterministic).  Leaves hold at
+// most 4 triangles.  Nodes are
 // stored in depth-first preorder, so an inner node's left child follows it,
 // and each node records its escape: the first node after its subtree.  The
 // device walks the tree without a stack (a missed box or a finished leaf
@@ -358,15 +364,27 @@ struct MeshBuild {
       if (it.e - it.b <= 4) {
         ub = 0x80000000u | ((it.e - it.b) << 27) | it.b;
       } else {
-        int ax = 0;
-        for (int a = 1; a < 3; a++)
-          if (ch[a] - cl[a] > ch[ax] - cl[ax]) ax = a;
-        const uint32_t mid = it.b + (it.e - it.b) / 2;
-        std::nth_element(order.begin() + it.b, order.begin() + mid, order.begin() + it.e,
-                         [&](uint32_t x, uint32_t y) {
-                           const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
-                           return cx < cy || (cx == cy && x < y);
-                         });
+        float nd2 = 0.0f;
+        for (int a = 0; a < 3; a++) nd2 += (bh[a] - bl[a]) * (bh[a] - bl[a]);
+        const auto large = [&](uint32_t k) {
+          float d2 = 0.0f;
+          for (int a = 0; a < 3; a++) d2 += (hi[3 * k + a] - lo[3 * k + a]) * (hi[3 * k + a] - lo[3 * k + a]);
+          return d2 > 0.25f * nd2;
+        };
+        uint32_t mid = (uint32_t)(std::stable_partition(order.begin() + it.b,
+                                                        order.begin() + it.e, large) -
+                                  order.begin());
+        if (mid == it.b || mid == it.e) {
+          int ax = 0;
+          for (int a = 1; a < 3; a++)
+            if (ch[a] - cl[a] > ch[ax] - cl[ax]) ax = a;
+          mid = it.b + (it.e - it.b) / 2;
+          std::nth_element(order.begin() + it.b, order.begin() + mid, order.begin() + it.e,
+                           [&](uint32_t x, uint32_t y) {
+                             const float cx = cen[3 * x + ax], cy = cen[3 * y + ax];
+                             return cx < cy || (cx == cy && x < y);
+                           });
+        }
         // Left child (node ni + 1) first: pushed last.
         stack.push_back({mid, it.e, 0u, false});
         stack.push_back({it.b, mid, 0u, false});
