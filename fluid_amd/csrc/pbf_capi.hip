@@ -226,7 +226,10 @@ int finish_create(Ctx* x, pbf_ctx** out) {
   // PBF_NO_SMALL_SORT keeps the multi-launch scan and sort for small states
   // too (same bits; for measurement).
   x->small_sort = getenv("PBF_NO_SMALL_SORT") == nullptr;
-  if (const char* w = getenv("PBF_WIDE_MAX")) x->wide_max = atoll(w);
+  if (const char* w = getenv("PBF_WIDE_MAX"))
+    for (pbf::PassRule& r : x->rules) r.wave_max = atoll(w);
+  if (const char* w = getenv("PBF_QUAD_MAX"))
+    for (int k : {pbf::kPassLambda, pbf::kPassDeltap, pbf::kPassXsph}) x->rules[k].quad_max = atoll(w);
   if (verify_math(x) != PBF_OK) {
     std::string m = x->err;
     (void)hipStreamDestroy(x->stream);

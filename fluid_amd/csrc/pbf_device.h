@@ -677,6 +677,73 @@ __device__ __forceinline__ void for_each_neighbor_pipe(uint32_t i, int cnt, int 
   }
 }
 
+// Four lanes per particle (one DPP quad).  Lane k = lane % 4 of particle
+// i's quad gathers entry 4c + k of every 4-entry chunk c and evaluates its
+// NS terms with term(j, data, t); then every lane of the quad adds the
+// chunk's four terms to sums[] in list order, taking its partners' terms
+// with DPP quad_perm broadcasts of lanes 0, 1, 2, 3.  Entries past the list
+// end in the last chunk (padding slots, which hold a particle index)
+// contribute +0: a sum that starts at +0 is never -0, and s + (+0) = s for
+// every other s, so the sums are those of the lane-per-particle loops bit
+// for bit.  A quad's lanes share i and cnt, so they run the loop in step.
+// The next chunk's gather and the one after's index load are in flight
+// while a chunk's terms are evaluated.
+__device__ __forceinline__ float quad_lane(float t, int k) {
+  int r;
+  switch (k) {  // the DPP control must be a constant
+    case 0: r = __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0x00, 0xf, 0xf, false); break;
+    case 1: r = __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0x55, 0xf, 0xf, false); break;
+    case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0xAA, 0xf, 0xf, false); break;
+    default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(t), 0xFF, 0xf, 0xf, false); break;
+  }
+  return __int_as_float(r);
+}
+
+template <int NS, class Load, class Term>
+__device__ __forceinline__ void quad_sums(uint32_t i, int cnt, int K4,
+                                          const uint32_t* __restrict__ nbr, Load load, Term term,
+                                          float* sums) {
+  using D = decltype(load(0u));
+  const int k = (int)(threadIdx.x & 3u);
+#pragma unroll
+  for (int q = 0; q < NS; q++) sums[q] = 0.0f;
+  auto idx = [&](int ch) { return load_chunk(nbr + nbr_chunk(i, ch, K4)); };
+  auto pick = [&](const uint4& e) { return k == 0 ? e.x : k == 1 ? e.y : k == 2 ? e.z : e.w; };
+  const int nch = (cnt + 3) >> 2;
+  uint32_t j = i, jn = i;
+  uint4 an = make_uint4(i, i, i, i);
+  D dj{};
+  if (nch > 0) {
+    j = pick(idx(0));
+    dj = load(j);
+  }
+  if (nch > 1) an = idx(1);
+  for (int ch = 0; ch < nch; ch++) {
+    D dn{};
+    uint4 ann = an;
+    if (ch + 1 < nch) {
+      jn = pick(an);
+      dn = load(jn);
+    }
+    if (ch + 2 < nch) ann = idx(ch + 2);
+    float t[NS];
+    term(j, dj, t);
+    if (4 * ch + k >= cnt) {
+#pragma unroll
+      for (int q = 0; q < NS; q++) t[q] = 0.0f;
+    }
+    // Entries 4ch + 0 .. 3, in order.
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+#pragma unroll
+      for (int q = 0; q < NS; q++) sums[q] = sums[q] + quad_lane(t[q], s);
+    }
+    j = jn;
+    dj = dn;
+    an = ann;
+  }
+}
+
 // Chooses the visit of the lists: pipelined or grouped by U.
 template <int U, class Load, class Body>
 __device__ __forceinline__ void visit_neighbors(uint32_t i, int cnt, int K4,

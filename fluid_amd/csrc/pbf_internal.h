@@ -54,12 +54,34 @@ struct DevState {
   uint8_t* meshFlag = nullptr;    // scene-grid cells near a BVH leaf box (MeshView::flag)
 };
 
-// States of at most this many particles run the list passes with one wave
-// per particle (PBF_WIDE_MAX overrides at context creation; 0 turns them
-// off).  Same bits either way.  Measured per step (4 iterations): 4,096
-// particles 0.079 ms against 0.257 per lane, 8,192 0.089 / 0.197, 16,384
-// 0.125 / 0.209, 32,768 0.200 / 0.220, 65,536 0.340 / 0.231.
-constexpr int64_t kWideMaxDefault = 32768;
+// Lanes per particle of each list pass, by state size: one wave (64 lanes)
+// for states of at most wave_max particles, one DPP quad (4 lanes; single
+// scenes, lambda / delta-p / XSPH only) up to quad_max, else one lane.
+// Same bits in every case.  The bounds sit between the sizes where the
+// kernel time per launch crosses over (rocprofv3 kernel traces of lattice
+// blocks at 0.5 h spacing, 4 iterations; microseconds per launch, wave /
+// quad / lane):
+//   build + first lambda  32,768: 33.3 / -   / 44.9   49,152: 45.8 / -    / 45.4
+//   lambda                16,384:  7.3 / 8.3 / 12.3   24,576:  9.8 / 9.0  / 12.5
+//   delta-p                8,192:  6.9 / 8.7 / 13.0   16,384: 10.1 / 8.9  / 12.9
+//   XSPH                  16,384:  8.2 / 9.3 / 14.1   24,576: 11.3 / 10.1 / 14.4
+//   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
+// and quads against one lane at 98,304: lambda 13.6 / 14.4, delta-p 13.8 /
+// 15.0, XSPH 17.0 / 18.1, at 131,072 slower than one lane for the step.
+// PBF_WIDE_MAX (every pass's wave_max) and PBF_QUAD_MAX (quad_max) override
+// them at context creation; 0 turns a variant off.
+enum ListPass : int { kPassBuild, kPassLambda, kPassDeltap, kPassXsph, kPassVort, kListPasses };
+struct PassRule {
+  int64_t wave_max;
+  int64_t quad_max;
+};
+constexpr PassRule kPassRules[kListPasses] = {
+    {40960, 0},       // build + first lambda
+    {20480, 114688},  // lambda
+    {12288, 114688},  // delta-p
+    {20480, 114688},  // XSPH
+    {12288, 0},       // vorticity (+ commit, + prediction)
+};
 
 struct Ctx {
   pbf_params p{};
@@ -94,7 +116,8 @@ struct Ctx {
   int fast_rcp = 0;                  // short reciprocal verified exact
   int fast_sqrt = 0;                 // short square root verified exact
   bool small_sort = true;            // single-workgroup scan + sort for small states
-  int64_t wide_max = kWideMaxDefault;  // wave-per-particle list passes up to this many particles
+  PassRule rules[kListPasses] = {kPassRules[0], kPassRules[1], kPassRules[2], kPassRules[3],
+                                 kPassRules[4]};
   // Density-field output on the device, kept between calls, and two pinned
   // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;

@@ -16,6 +16,9 @@
 //                  call fused with the next step's predict
 // Particles stay in the order (cell, id), so the lists of a particle are in
 // ascending sorted index, i.e. the same order a CPU visits the 27 cells in.
+// The list passes come with one lane, one quad (4 lanes) or one wave per
+// particle, chosen per pass by state size (PassRule in pbf_internal.h); each
+// adds its terms in list order from +0, so all three give the same bits.
 //
 // Every pass is a per-particle device function.  The flat schedule runs one
 // launch per pass over all particles of all scenes; the persistent schedule
@@ -1220,6 +1223,67 @@ __global__ __launch_bounds__(kWide) void k_vort_wide(Dev d, SimConst c, SceneGri
   }
 }
 
+// ---- quad list passes: four lanes per particle ------------------------------
+// With one lane per particle a state of some ten thousand particles leaves
+// the chip with few waves, each walking its particles' lists one entry
+// after another; with one wave per particle every 64-entry block of a list
+// costs a sequential sum in one lane.  With a quad per particle (quad_sums)
+// there are four times the waves of the lane-per-particle passes and each
+// walks a quarter of the list; the sums are the same bits.
+__device__ __forceinline__ uint32_t quad_index() {
+  return (xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) >> 2;
+}
+
+template <bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_lambda_quad(Dev d, SimConst c, const float4* in,
+                                                        float4* out) {
+  const uint32_t i = quad_index();
+  if (i >= d.n) return;  // a quad's lanes leave together
+  const float3 xi = PACKED ? d.xs3[i] : make_float3(in[i].x, in[i].y, in[i].z);
+  float s[5];
+  quad_sums<5>(
+      i, (int)d.ncnt[i], d.K4, d.nbr,
+      [&](uint32_t j) {
+        if (PACKED) return at(d.xs3, j);
+        const float4 q = at(in, j);
+        return make_float3(q.x, q.y, q.z);
+      },
+      [&](uint32_t, const float3& xj, float* t) { lambda_terms(c, xi, xj, t); }, s);
+  if ((threadIdx.x & 3u) == 0)
+    out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, s[0], s[4], s[1], s[2], s[3]));
+}
+
+template <bool LAST, int NT>
+__global__ __launch_bounds__(kBlock) void k_deltap_quad(Dev d, SimConst c, SceneGrid g,
+                                                        const float4* in) {
+  const uint32_t i = quad_index();
+  if (i >= d.n) return;
+  const float4 xi = in[i];
+  float s[3];
+  quad_sums<3>(
+      i, (int)d.ncnt[i], d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
+      [&](uint32_t, const float4& xj, float* t) { deltap_terms<NT>(c, xi, xj, t); }, s);
+  if ((threadIdx.x & 3u) == 0) deltap_finish<LAST>(i, d, c, g, xi, s[0], s[1], s[2]);
+}
+
+struct XVRec {
+  float3 x, v;
+};
+
+__global__ __launch_bounds__(kBlock) void k_xsph_quad(Dev d, SimConst c, float4* out) {
+  const uint32_t i = quad_index();
+  if (i >= d.n) return;
+  const float3* __restrict__ xv = reinterpret_cast<const float3*>(d.xv);
+  const float3 xi = xv[2 * (size_t)i], vi = xv[2 * (size_t)i + 1];
+  float s[7];
+  quad_sums<7>(
+      i, (int)d.ncnt[i], d.K4, d.nbr,
+      [&](uint32_t j) { return XVRec{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
+      [&](uint32_t, const XVRec& q, float* t) { xsph_terms(c, xi, vi, q.x, q.v, t); }, s);
+  if ((threadIdx.x & 3u) == 0)
+    xsph_finish(i, d, c, xi, vi, s[0], s[1], s[2], s[3], s[4], s[5], s[6], out);
+}
+
 // Scan, scatter and rank of a small state in one workgroup: the three
 // passes that otherwise take four or five launches, with workgroup barriers
 // between them (every thread of the workgroup shares one CU's cache, so a
@@ -1396,12 +1460,22 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
   return hipGetLastError();
 }
 
-static bool wide(const Ctx& x) { return x.n <= x.wide_max; }
+// Lanes per particle of list pass p: 64 (a wave: k_*_wide), 4 (a quad:
+// k_*_quad, single scenes) or 1.
+static int lanes(const Ctx& x, ListPass p) {
+  const PassRule& r = x.rules[p];
+  if (x.n <= r.wave_max) return 64;
+  if (x.nscenes == 1 && x.n <= r.quad_max) return 4;
+  return 1;
+}
+static bool wide(const Ctx& x, ListPass p) { return lanes(x, p) == 64; }
+static bool quad(const Ctx& x, ListPass p) { return lanes(x, p) == 4; }
 static unsigned wide_grid(const Ctx& x) { return (unsigned)x.n; }
+static unsigned quad_grid(const Ctx& x) { return blocks_for(x.n * 4); }
 
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   const Dev d = dev_of(x);
-  if (wide(x)) {
+  if (wide(x, kPassBuild)) {
     if (multi(x))
       k_nbr_lambda_wide<true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], out);
     else
@@ -1416,7 +1490,14 @@ hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
 }
 
 hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* out, bool packed) {
-  if (wide(x)) {
+  if (quad(x, kPassLambda)) {
+    if (packed)
+      k_lambda_quad<true><<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+    else
+      k_lambda_quad<false><<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+    return hipGetLastError();
+  }
+  if (wide(x, kPassLambda)) {
     if (packed)
       k_lambda_wide<true><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, in, out);
     else
@@ -1434,7 +1515,14 @@ template <int NT>
 static void launch_deltap_nt(const Ctx& x, const Dev& d, const SimConst& c, const float4* in,
                              float4* out, bool last) {
   const unsigned nb = blocks_for(x.n);
-  if (wide(x)) {
+  if (quad(x, kPassDeltap)) {
+    if (last)
+      k_deltap_quad<true, NT><<<quad_grid(x), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
+    else
+      k_deltap_quad<false, NT><<<quad_grid(x), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
+    return;
+  }
+  if (wide(x, kPassDeltap)) {
     const unsigned ng = wide_grid(x);
     if (multi(x)) {
       if (last)
@@ -1472,7 +1560,11 @@ hipError_t launch_deltap(Ctx& x, const SimConst& c, const float4* in, float4* ou
 }
 
 hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
-  if (wide(x)) {
+  if (quad(x, kPassXsph)) {
+    k_xsph_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, out);
+    return hipGetLastError();
+  }
+  if (wide(x, kPassXsph)) {
     k_xsph_wide<<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, out);
     return hipGetLastError();
   }
@@ -1494,7 +1586,7 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
   // After the single-workgroup sort the histogram is already zero.
   const bool zeroed = x.small_sort && sort_in_one_workgroup(x);
   const Dev d = dev_of(x);
-  if (wide(x) && !zeroed) {
+  if (wide(x, kPassVort) && !zeroed) {
     // Beyond the smallest states the wide pass commits and the prediction
     // follows as its own launch (the serial tail of one lane per particle,
     // with one histogram atomic each, costs more than the launch).  It is
@@ -1513,7 +1605,7 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
     const hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
     if (e != hipSuccess) return e;
   }
-  if (wide(x)) {
+  if (wide(x, kPassVort)) {
     if (multi(x))
       k_vort_wide<true, true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
     else
@@ -1528,7 +1620,7 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
 }
 
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
-  if (wide(x)) {
+  if (wide(x, kPassVort)) {
     k_vort_wide<false, false><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, x.grids[0], in);
     return hipGetLastError();
   }

@@ -30,6 +30,8 @@ checkpoints every 10 steps, loss = mean |x_T - target|^2.
   segment's replay recomputes the forward steps exactly and the adjoint
   kernels are deterministic).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -139,23 +141,38 @@ def test_checkpoint_spacing_gives_the_same_bits(full):
         assert kb == ref["kb"], every
 
 
-def test_lane_per_particle_rollout(orc):
+def test_mid_size_rollout(orc):
     """A 36^3 block (46,656 particles: above the wave-per-particle range, so
-    the forward pass and its replays run the lane-per-particle passes), 5
+    the forward pass and its replays run a quad per particle in lambda,
+    delta-p and XSPH and a lane in the build and the vorticity pass), 5
     steps with checkpoints every 2: forward bit-exact with the oracle, the
-    replay equal to the forward pass, and FP64 central differences along
-    dL/dv0 and one random direction within 1e-3."""
+    replay equal to the forward pass, FP64 central differences along dL/dv0
+    and one random direction within 1e-3, and a rollout created with
+    PBF_QUAD_MAX=0 (one lane per particle throughout) gives the same bits
+    for the forward pass and the gradients."""
     sc, target = scenes.diff_block(n_side=36, steps=5)
     assert sc.n == 46_656
     k = float(np.float32(sc.params.k_tensile))
     r = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=2)
     xT, vT = r.forward(sc.x, sc.v, k)
     xbar = ((2.0 / sc.n) * (xT.astype(np.float64) - target)).astype(np.float32)
-    _, vb0, kb = r.backward(xbar)
+    xb0, vb0, kb = r.backward(xbar)
     assert r.replay_mismatches() == 0
     _, contacts = r.stats()
     r.close()
     assert contacts == 0
+    os.environ["PBF_QUAD_MAX"] = "0"
+    try:
+        r1 = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=2)
+    finally:
+        os.environ.pop("PBF_QUAD_MAX", None)
+    xT1, vT1 = r1.forward(sc.x, sc.v, k)
+    xb1, vb1, kb1 = r1.backward(xbar)
+    assert r1.replay_mismatches() == 0
+    r1.close()
+    for a, b in ((xT, xT1), (vT, vT1), (xb0, xb1), (vb0, vb1)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert kb == kb1
     ox, ov, _ = orc.step(sc.params, sc.x, sc.v, sc.steps, orc.NBR_GRID)
     assert np.array_equal(xT.view(np.uint32), ox.view(np.uint32))
     assert np.array_equal(vT.view(np.uint32), ov.view(np.uint32))

@@ -98,10 +98,15 @@ def test_dam_break_262k_steps_bit_exact(orc):
     assert bits_equal(hx, ox) and bits_equal(hv, ov)
 
 
-def test_lane_per_particle_100_steps_bit_exact(orc):
-    """100 steps of a 40,000-particle block with random velocities (the
-    lane-per-particle passes; walls reached on every side) in one call:
-    the oracle's bits, so the 100-step drift of this path is 0 too."""
+@pytest.mark.parametrize("quad_max", [None, "0"])
+def test_quad_and_lane_passes_100_steps_bit_exact(orc, quad_max):
+    """100 steps of a 40,000-particle block with random velocities (walls
+    reached on every side) in one call, with the default rule (a quad per
+    particle in lambda, delta-p and XSPH, a lane in the build and the
+    vorticity pass) and with one lane throughout (PBF_QUAD_MAX=0): the
+    oracle's bits, so the 100-step drift of both paths is 0 too."""
+    import os
+
     from fluid_amd import params as P
 
     p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
@@ -109,10 +114,15 @@ def test_lane_per_particle_100_steps_bit_exact(orc):
     x = scenes.lattice_block(40, 40, 25, 0.05, (0.1, 0.1, 0.1)).astype(np.float32)
     x += rng.uniform(-1e-3, 1e-3, x.shape).astype(np.float32)
     v = rng.uniform(-1.0, 1.0, x.shape).astype(np.float32)
-    with Particles(p) as c:
-        c.load(x, v)
-        c.time_step(nsteps=100)
-        gx, gv, gr = c.read()
+    if quad_max is not None:
+        os.environ["PBF_QUAD_MAX"] = quad_max
+    try:
+        with Particles(p) as c:
+            c.load(x, v)
+            c.time_step(nsteps=100)
+            gx, gv, gr = c.read()
+    finally:
+        os.environ.pop("PBF_QUAD_MAX", None)
     ox, ov, orho = orc.step(p, x, v, 100, orc.NBR_GRID)
     assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(gr, orho)
     lo = p.box_lo[0] + 0.05

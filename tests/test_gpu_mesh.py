@@ -75,10 +75,14 @@ def test_falling_block_on_sphere_bit_exact(orc_mesh):
     assert r.min() > 0.18 and (r < 0.25).sum() > 50
 
 
-def test_falling_slab_on_sphere_lane_per_particle_bit_exact(orc_mesh):
-    """A 48x20x48 slab (46,080 particles, above the wave-per-particle range,
-    so the lane-per-particle passes and their collision calls run) falling
-    onto the sphere: the oracle's bits after 15 and 30 steps."""
+def test_falling_slab_on_sphere_quad_and_lane_passes_bit_exact(orc_mesh):
+    """A 48x20x48 slab (46,080 particles, above the wave-per-particle range)
+    falling onto the sphere: the oracle's bits after 15 and 30 steps, with
+    the delta-p pass and its collision calls run with a quad per particle
+    (the default rule at this size) and, in a second context created with
+    PBF_QUAD_MAX=0, with one lane per particle."""
+    import os
+
     orc = orc_mesh
     rng = np.random.default_rng(5)
     s = 0.02
@@ -90,15 +94,23 @@ def test_falling_slab_on_sphere_lane_per_particle_bit_exact(orc_mesh):
     m = m + (mesh.default_skin(m[0], (0, 0, 0), (1, 1, 1)),)
     orc.set_mesh(*m)
     v = np.zeros_like(x)
-    with Particles(p) as c:
-        c.set_mesh(*m)
-        c.load(x, v)
+    os.environ["PBF_QUAD_MAX"] = "0"
+    try:
+        lane = Particles(p)
+    finally:
+        os.environ.pop("PBF_QUAD_MAX", None)
+    with Particles(p) as c, lane:
+        for ctx in (c, lane):
+            ctx.set_mesh(*m)
+            ctx.load(x, v)
         ox, ov = x, v
         for _ in range(2):
             c.time_step(nsteps=15)
+            lane.time_step(nsteps=15)
             ox, ov, _ = orc.step(p, ox, ov, 15, orc.NBR_GRID)
-            gx, gv, _ = c.read()
-            assert bits_equal(gx, ox) and bits_equal(gv, ov)
+            for ctx in (c, lane):
+                gx, gv, _ = ctx.read()
+                assert bits_equal(gx, ox) and bits_equal(gv, ov)
     r = np.linalg.norm(gx - np.array([0.5, 0.3, 0.5]), axis=1)
     assert r.min() > 0.18 and (r < 0.25).sum() > 100  # on the sphere, none inside it
 

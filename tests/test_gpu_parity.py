@@ -224,16 +224,35 @@ def test_small_state_single_workgroup_sort_over_several_scan_chunks(orc):
         assert bits_equal(gx, xg) and bits_equal(gv, vg)
 
 
-@pytest.mark.parametrize("spacing_over_h", [0.5, 0.3])
-def test_wide_passes_equal_per_lane_passes(orc, spacing_over_h):
-    """States of up to 32,768 particles run the list passes with one wave per
-    particle; contexts created with PBF_WIDE_MAX=0 run the per-lane passes.
-    Same bits for the dam block (about 31 neighbours) and for a block at
-    spacing 0.3 h (about 150 neighbours at first, 98 at most after three
-    steps: lists of several 64-entry blocks),
-    and both equal the oracle."""
-    import os
+LANE_MODES = (
+    {},  # the default rule: a wave per particle for small states
+    {"PBF_WIDE_MAX": "0"},  # quads for lambda / delta-p / XSPH, lanes for the rest
+    {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0"},  # one lane per particle throughout
+)
 
+
+def run_modes(p, x, v, nsteps, modes=LANE_MODES):
+    """(read(), neighbor_stats()) after nsteps steps, per lane mode."""
+    outs = []
+    for env in modes:
+        os.environ.update(env)
+        try:
+            with ctx_for(p, x, v) as c:
+                c.time_step(nsteps=nsteps)
+                outs.append((c.read(), c.neighbor_stats()))
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+    return outs
+
+
+@pytest.mark.parametrize("spacing_over_h", [0.5, 0.3])
+def test_wide_quad_and_lane_passes_equal(orc, spacing_over_h):
+    """The list passes with one wave, one quad (4 lanes) or one lane per
+    particle give the same bits, equal to the oracle's, for the dam block
+    (about 31 neighbours) and for a block at spacing 0.3 h (about 150
+    neighbours at first, 98 at most after three steps: lists of several
+    64-entry blocks and of chunks past the end of a quad's last chunk)."""
     sc = scenes.dam_block()
     h = sc.params.h
     s = spacing_over_h * h
@@ -242,20 +261,25 @@ def test_wide_passes_equal_per_lane_passes(orc, spacing_over_h):
     v = np.random.default_rng(5).uniform(-0.2, 0.2, x.shape).astype(np.float32)
     p = sc.params.replace(max_neighbors=512)
     xg, vg, rg = orc.step(p, x, v, 3, orc.NBR_GRID)
-    outs = []
-    for env in ({}, {"PBF_WIDE_MAX": "0"}):
-        os.environ.update(env)
-        try:
-            with ctx_for(p, x, v) as c:
-                c.time_step(nsteps=3)
-                outs.append(c.read())
-                nbar, nmax = c.neighbor_stats()
-        finally:
-            for k in env:
-                os.environ.pop(k, None)
+    outs = run_modes(p, x, v, 3)
     if spacing_over_h < 0.4:
-        assert nmax > 64  # several 64-entry blocks per list
-    for gx, gv, gr in outs:
+        assert outs[0][1][1] > 64  # several 64-entry blocks per list
+    for (gx, gv, gr), _ in outs:
+        assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
+
+
+def test_mid_size_state_quad_passes_bit_exact(orc):
+    """49,152 particles, where the default rule runs lambda, delta-p and XSPH
+    with a quad per particle and the build and vorticity with a lane: five
+    steps equal the oracle's, and the one-lane passes give the same bits."""
+    s = 0.05
+    x = scenes.lattice_block(48, 32, 32, s, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += np.random.default_rng(1).uniform(-1e-3, 1e-3, x.shape).astype(np.float32)
+    v = np.random.default_rng(2).uniform(-0.5, 0.5, x.shape).astype(np.float32)
+    p = P.scene_params(0.1, s, (0, 0, 0), (4.0, 4.0, 3.0), dt=0.01)
+    xg, vg, rg = orc.step(p, x, v, 5, orc.NBR_GRID)
+    outs = run_modes(p, x, v, 5, (LANE_MODES[0], LANE_MODES[2]))
+    for (gx, gv, gr), _ in outs:
         assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
 
 

@@ -43,10 +43,11 @@ def test_resume_is_bit_exact(tmp_path):
             assert bits_equal(a, b)
 
 
-def test_resume_lane_per_particle_state_is_bit_exact(tmp_path):
-    """40,000 particles (lane-per-particle passes, whose last pass of a step
-    also predicts the next step when the call goes on): 12 steps in one call
-    against 6, a snapshot, a fresh context and 6 more."""
+def test_resume_mid_size_state_is_bit_exact(tmp_path):
+    """40,000 particles (a lane per particle in the vorticity pass, which
+    also predicts the next step when the call goes on; a quad per particle
+    in lambda, delta-p and XSPH): 12 steps in one call against 6, a
+    snapshot, a fresh context and 6 more."""
     from fluid_amd import params as P
 
     p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)

@@ -87,11 +87,13 @@ def test_noalloc_matches_allocating_path_on_two_calls():
             assert not same_bits(gx, np.asarray(x_in, np.float32))
 
 
-def test_noalloc_lane_per_particle_state():
-    """A 40,000-particle block (above the wave-per-particle range, so the
-    lane-per-particle passes run; within the 64 MiB workspace at 256
-    neighbours), then every second particle (20,000: wave-per-particle
-    passes) in the same workspace: the allocating path's bits both times."""
+def test_noalloc_mid_size_states():
+    """A 40,000-particle block (one lane per particle in the build and the
+    vorticity pass, a quad in lambda, delta-p and XSPH; within the 64 MiB
+    workspace at 256 neighbours), then every second particle (20,000: a
+    wave in the build, lambda and XSPH, a quad in delta-p, a lane in the
+    vorticity pass) in the same workspace: the allocating path's bits both
+    times."""
     from fluid_amd import params as P
 
     p = P.scene_params(0.1, 0.05, (0.0, 0.0, 0.0), (3.0, 3.0, 3.0), dt=0.01)
