@@ -492,7 +492,7 @@ __device__ __forceinline__ uint4 load_chunk(const uint32_t* __restrict__ p) {
 #define PBF_XCD_SWIZZLE 1
 #endif
 constexpr uint32_t kXcdRun = 256;
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+__host__ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   if (!PBF_XCD_SWIZZLE) return b;
   const uint32_t q = nb >> 3;
   if (q > kXcdRun) {

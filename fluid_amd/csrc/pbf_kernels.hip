@@ -1465,7 +1465,8 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
 static int lanes(const Ctx& x, ListPass p) {
   const PassRule& r = x.rules[p];
   if (x.n <= r.wave_max) return 64;
-  if (x.nscenes == 1 && x.n <= r.quad_max) return 4;
+  // Quad kernels index 4n threads in 32 bits.
+  if (x.nscenes == 1 && x.n <= r.quad_max && x.n < (int64_t{1} << 30)) return 4;
   return 1;
 }
 static bool wide(const Ctx& x, ListPass p) { return lanes(x, p) == 64; }

@@ -9,7 +9,10 @@
   (tests/sanitize/oracle_checks.c);
 * the C++ driver's XML parsing on missing, foreign and malformed files
   (fluid_amd/host/pbf_driver.cpp; with no device the run stops at context
-  creation, after the parse).
+  creation, after the parse);
+* the list passes' workgroup order (xcd_block of pbf_device.h, built by
+  hipcc and run on the host): a permutation of the workgroups for every
+  grid size, with the documented runs per XCD (tests/sanitize/xcd_order.hip).
 
 Each is compiled here with -fsanitize=address,undefined and
 -fno-sanitize-recover, so any finding fails the run; the host checks are
@@ -110,3 +113,15 @@ def test_driver_parsing_under_sanitizers(tmp_path):
     r = run([exe, "-p", os.path.join(ROOT, "tests", "golden", "p.xml"), "-d", "0.02", "-q"])
     assert r.returncode in (0, 1), r.stderr
     assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None, reason="no hipcc")
+def test_xcd_block_order_is_a_permutation(tmp_path):
+    exe = str(tmp_path / "xcd_order")
+    build(["hipcc", "-std=c++17", "--offload-arch=gfx950", "-g", "-O1",
+           "-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-fno-sanitize-recover=all",
+           "-I", os.path.join(ROOT, "fluid_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "sanitize", "xcd_order.hip"), "-o", exe])
+    r = run([exe])
+    assert r.returncode == 0 and "all grids checked" in r.stdout, r.stdout + r.stderr
+    assert "runtime error" not in r.stderr
