@@ -3,7 +3,11 @@ oracle: an isolated particle (semi-implicit Euler with the reference's
 velocity update, src/particles.cpp:175-183 and :215-217), pairs at exactly
 r = h (the neighbour test is inclusive, :258-265), particle counts around
 the wave and tile sizes (1 .. 257), particles starting on the walls, and
-the neighbour-list capacity at and one chunk below the largest list."""
+the neighbour-list capacity at and one chunk below the largest list.  The
+step cases run with each variant of the list passes: a wave, a quad (4
+lanes) or a lane per particle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -21,14 +25,32 @@ def bits_equal(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def run(p, x, v, nsteps):
-    with Particles(p) as c:
+# Context environments that select the list-pass variant for these small
+# states: the default rule (a wave per particle), quads for lambda, delta-p
+# and XSPH (lanes for the build and vorticity), one lane throughout.
+MODES = {
+    "wave": {},
+    "quad": {"PBF_WIDE_MAX": "0"},
+    "lane": {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0"},
+}
+
+
+def run(p, x, v, nsteps, mode="wave"):
+    env = MODES[mode]
+    os.environ.update(env)
+    try:
+        c = Particles(p)
+    finally:
+        for k in env:
+            os.environ.pop(k, None)
+    with c:
         c.load(x, v)
         c.time_step(nsteps=nsteps)
         return c.read()
 
 
-def test_isolated_particle_is_semi_implicit_euler(orc):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_isolated_particle_is_semi_implicit_euler(orc, mode):
     """No neighbours: rho = 0, delta-p = 0, XSPH and confinement add
     nothing, so each step is v.y -= g dt, x* = x + v dt, v = (x* - x) / dt,
     x = x*, in float32 in that order."""
@@ -36,7 +58,7 @@ def test_isolated_particle_is_semi_implicit_euler(orc):
     x = np.array([[0.5, 0.9, 0.5]], np.float32)
     v = np.array([[0.3, 0.0, -0.2]], np.float32)
     steps = 20
-    gx, gv, grho = run(p, x, v, steps)
+    gx, gv, grho = run(p, x, v, steps, mode)
     dt = f32(p.dt)
     gdt = f32(p.gravity) * dt
     inv_dt = f32(1.0) / dt
@@ -52,7 +74,8 @@ def test_isolated_particle_is_semi_implicit_euler(orc):
     assert bits_equal(gx, ox) and bits_equal(gv, ov)
 
 
-def test_pairs_at_exactly_h_are_neighbours(orc):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_pairs_at_exactly_h_are_neighbours(orc, mode):
     """h = 1/8 so that h^2 is exact; with no gravity the predicted positions
     equal the loaded ones.  B and D sit exactly h from A (included), C one
     float above h (excluded)."""
@@ -73,26 +96,29 @@ def test_pairs_at_exactly_h_are_neighbours(orc):
     og, lg = orc.neighbors(p, x, orc.NBR_GRID)
     for i in range(4):
         assert np.array_equal(ids[i, :counts[i]], lg[og[i]:og[i + 1]])
-    gx, gv, grho = run(p, x, v, 3)
+    gx, gv, grho = run(p, x, v, 3, mode)
     ox, ov, orho = orc.step(p, x, v, 3, orc.NBR_GRID)
     assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
     # poly6 and the spiky gradient vanish at r = h: A's density is 0.
     assert grho[0] == 0.0
 
 
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 129, 257])
-def test_ragged_counts_bit_exact(orc, n):
-    """Counts around the 64-lane wave and the 256-particle workgroup."""
+def test_ragged_counts_bit_exact(orc, n, mode):
+    """Counts around the 64-lane wave and the 256-particle workgroup (and,
+    with quads, the 64-particle quad workgroup)."""
     sc = scenes.dam_block()
     rng = np.random.default_rng(n)
     x = sc.x[rng.choice(len(sc.x), size=n, replace=False)].copy()
     v = rng.uniform(-0.5, 0.5, size=x.shape).astype(np.float32)
-    gx, gv, grho = run(sc.params, x, v, 3)
+    gx, gv, grho = run(sc.params, x, v, 3, mode)
     ox, ov, orho = orc.step(sc.params, x, v, 3, orc.NBR_GRID)
     assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
 
 
-def test_particles_on_the_walls_bit_exact(orc):
+@pytest.mark.parametrize("mode", list(MODES))
+def test_particles_on_the_walls_bit_exact(orc, mode):
     """Particles loaded exactly on the faces, edges and corners of the box,
     moving outwards: ray clipping, the slide and the per-axis clamp."""
     sc = scenes.dam_block()
@@ -111,7 +137,7 @@ def test_particles_on_the_walls_bit_exact(orc):
         vel.append(w)
     x = np.array(pts, np.float32)
     v = np.array(vel, np.float32)
-    gx, gv, grho = run(sc.params, x, v, 4)
+    gx, gv, grho = run(sc.params, x, v, 4, mode)
     ox, ov, orho = orc.step(sc.params, x, v, 4, orc.NBR_GRID)
     assert bits_equal(gx, ox) and bits_equal(gv, ov) and bits_equal(grho, orho)
     assert np.all(gx >= lo) and np.all(gx <= hi)
