@@ -56,27 +56,30 @@ struct DevState {
 
 // Lanes per particle of each list pass, by state size: one wave (64 lanes)
 // for states of at most wave_max particles, one DPP quad (4 lanes; single
-// scenes, lambda / delta-p / XSPH only) up to quad_max, else one lane.
+// scenes; not the vorticity pass) up to quad_max, else one lane.
 // Same bits in every case.  The bounds sit between the sizes where the
 // kernel time per launch crosses over (rocprofv3 kernel traces of lattice
 // blocks at 0.5 h spacing, 4 iterations; microseconds per launch, wave /
 // quad / lane):
-//   build + first lambda  32,768: 33.3 / -   / 44.9   49,152: 45.8 / -    / 45.4
+//   build + first lambda  16,384: 20.7 / 26.0 / -     24,576: 27.3 / 26.9 / -
+//                         32,768: 34.3 / 27.3 / 44.9  98,304: -    / 43.4 / 52.1
 //   lambda                16,384:  7.3 / 8.3 / 12.3   24,576:  9.8 / 9.0  / 12.5
 //   delta-p                8,192:  6.9 / 8.7 / 13.0   16,384: 10.1 / 8.9  / 12.9
 //   XSPH                  16,384:  8.2 / 9.3 / 14.1   24,576: 11.3 / 10.1 / 14.4
 //   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
 // and quads against one lane at 98,304: lambda 13.6 / 14.4, delta-p 13.8 /
-// 15.0, XSPH 17.0 / 18.1, at 131,072 slower than one lane for the step.
-// PBF_WIDE_MAX (every pass's wave_max) and PBF_QUAD_MAX (quad_max) override
-// them at context creation; 0 turns a variant off.
+// 15.0, XSPH 17.0 / 18.1; at 131,072 the build 56.3 / 55.0, and the step
+// slower with quads.  (The quad build's figure includes its separate first
+// lambda launch.)  PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX
+// (quad_max) and PBF_RULE_<PASS>=wave_max,quad_max override them at context
+// creation; 0 turns a variant off.
 enum ListPass : int { kPassBuild, kPassLambda, kPassDeltap, kPassXsph, kPassVort, kListPasses };
 struct PassRule {
   int64_t wave_max;
   int64_t quad_max;
 };
 constexpr PassRule kPassRules[kListPasses] = {
-    {40960, 0},       // build + first lambda
+    {24576, 114688},  // build + first lambda
     {20480, 114688},  // lambda
     {12288, 114688},  // delta-p
     {20480, 114688},  // XSPH

@@ -1266,6 +1266,88 @@ __global__ __launch_bounds__(kBlock) void k_deltap_quad(Dev d, SimConst c, Scene
   if ((threadIdx.x & 3u) == 0) deltap_finish<LAST>(i, d, c, g, xi, s[0], s[1], s[2]);
 }
 
+// The neighbour build with a quad per particle (single scene; nbr_one's
+// rows, pruning and order).  The quad walks each row 16 candidates at a
+// time: lane k tests candidates k, k + 4, k + 8, k + 12, and four DPP
+// broadcasts give every lane the quad's 16-bit hit mask in candidate order.
+// A hit lane stores its entry at slot cnt + (hits before it in the mask),
+// so the list is nbr_one's.  The slots past the end of the last chunk hold
+// i (a particle index, as the readers' gathers need; their terms are not
+// added).  The first lambda pass follows as its own quad launch.
+__device__ __forceinline__ uint32_t quad_or(uint32_t m) {
+  return (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 0)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 1)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 2)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 3));
+}
+
+__global__ __launch_bounds__(kBlock) void k_nbr_quad(Dev d, float h2, SceneGrid g) {
+  const uint32_t i = quad_index();
+  if (i >= d.n) return;  // a quad's lanes leave together
+  const uint32_t k = threadIdx.x & 3u;
+  const float4 xi = d.predQ[i];
+  int cc[3];
+  cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
+  const float cell = 1.0f / g.inv_cell;
+  const float hm2 = h2 * 1.0201f;  // (1.01 h)^2
+  const float4* __restrict__ pred = d.predQ;
+  const uint32_t cap = (uint32_t)d.K4 * kChunk;
+  auto hit = [&](const float4& xj) -> uint32_t {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    return d2 <= h2 ? 1u : 0u;
+  };
+  auto put = [&](uint32_t slot, uint32_t j) {
+    if (slot < cap) d.nbr[nbr_chunk(i, (int)(slot >> 2), d.K4) + (slot & 3u)] = j;
+  };
+  uint32_t cnt = 0;
+  const float gxl = face_gap(xi.x, g.lo[0], cell, cc[0], -1);
+  const float gxr = face_gap(xi.x, g.lo[0], cell, cc[0], 1);
+  for (int dz = -1; dz <= 1; dz++) {
+    const int z = cc[2] + dz;
+    if (z < 0 || z >= g.dims[2]) continue;
+    const float gz = face_gap(xi.z, g.lo[2], cell, cc[2], dz);
+    for (int dy = -1; dy <= 1; dy++) {
+      const int y = cc[1] + dy;
+      if (y < 0 || y >= g.dims[1]) continue;
+      const float gy = face_gap(xi.y, g.lo[1], cell, cc[1], dy);
+      const float yz2 = gy * gy + gz * gz;
+      if (yz2 > hm2) continue;
+      const int x0 = (cc[0] > 0 && gxl * gxl + yz2 <= hm2) ? cc[0] - 1 : cc[0];
+      const int x1 = (cc[0] + 1 < g.dims[0] && gxr * gxr + yz2 <= hm2) ? cc[0] + 1 : cc[0];
+      const uint32_t row =
+          g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
+      const uint32_t s = d.cellStart[row + (uint32_t)x0];
+      const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
+      for (uint32_t jb = s; jb < e; jb += 16) {
+        // Up to 15 candidates past the range are read (the x* arrays are
+        // padded by 64 entries); their bits are dropped.
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          const uint32_t j = jb + k + 4u * u;
+          mine |= (hit(at(pred, j)) & (uint32_t)(j < e && j != i)) << (4u * u + k);
+        }
+        const uint32_t m = quad_or(mine);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          const uint32_t t = 4u * u + k;
+          if ((m >> t) & 1u) put(cnt + (uint32_t)__builtin_popcount(m & ((1u << t) - 1u)), jb + t);
+        }
+        cnt += (uint32_t)__builtin_popcount(m);
+      }
+    }
+  }
+  // Slots cnt .. end of its chunk: lane k fills slot cnt + k if that is in
+  // the same chunk.
+  const uint32_t pad = cnt + k;
+  if ((cnt & 3u) && (pad >> 2) == (cnt >> 2)) put(pad, i);
+  if (k == 0) {
+    if (cnt > cap) atomicMax(&d.flags[0], cnt);
+    d.ncnt[i] = min(cnt, cap);
+  }
+}
+
 struct XVRec {
   float3 x, v;
 };
@@ -1451,15 +1533,6 @@ hipError_t launch_scatter_rank(Ctx& x, const float4* pred) {
   return hipGetLastError();
 }
 
-hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
-  const Dev d = dev_of(x);
-  if (multi(x))
-    k_nbr_build<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
-  else
-    k_nbr_build<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
-  return hipGetLastError();
-}
-
 // Lanes per particle of list pass p: 64 (a wave: k_*_wide), 4 (a quad:
 // k_*_quad, single scenes) or 1.
 static int lanes(const Ctx& x, ListPass p) {
@@ -1474,8 +1547,27 @@ static bool quad(const Ctx& x, ListPass p) { return lanes(x, p) == 4; }
 static unsigned wide_grid(const Ctx& x) { return (unsigned)x.n; }
 static unsigned quad_grid(const Ctx& x) { return blocks_for(x.n * 4); }
 
+hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
+  const Dev d = dev_of(x);
+  if (quad(x, kPassBuild)) {
+    k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
+    return hipGetLastError();
+  }
+  if (multi(x))
+    k_nbr_build<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
+  else
+    k_nbr_build<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
+  return hipGetLastError();
+}
+
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   const Dev d = dev_of(x);
+  if (quad(x, kPassBuild)) {
+    // The quad build, then the first lambda pass over the finished lists.
+    k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
+    k_lambda_quad<false><<<quad_grid(x), kBlock, 0, x.stream>>>(d, c, x.d.predQ, out);
+    return hipGetLastError();
+  }
   if (wide(x, kPassBuild)) {
     if (multi(x))
       k_nbr_lambda_wide<true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], out);

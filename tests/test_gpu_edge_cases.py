@@ -35,15 +35,18 @@ MODES = {
 }
 
 
-def run(p, x, v, nsteps, mode="wave"):
+def context(p, mode="wave"):
     env = MODES[mode]
     os.environ.update(env)
     try:
-        c = Particles(p)
+        return Particles(p)
     finally:
         for k in env:
             os.environ.pop(k, None)
-    with c:
+
+
+def run(p, x, v, nsteps, mode="wave"):
+    with context(p, mode) as c:
         c.load(x, v)
         c.time_step(nsteps=nsteps)
         return c.read()
@@ -87,7 +90,7 @@ def test_pairs_at_exactly_h_are_neighbours(orc, mode):
         [0.5, 0.5, 0.375],                                    # D: dz = -h
     ], np.float32)
     v = np.zeros_like(x)
-    with Particles(p) as c:
+    with context(p, mode) as c:
         c.load(x, v)
         c.step_debug("neighbors")
         counts, ids = c.neighbors(stride=8)
@@ -143,7 +146,10 @@ def test_particles_on_the_walls_bit_exact(orc, mode):
     assert np.all(gx >= lo) and np.all(gx <= hi)
 
 
-def test_list_capacity_at_and_below_the_largest_list():
+@pytest.mark.parametrize("mode", list(MODES))
+def test_list_capacity_at_and_below_the_largest_list(mode):
+    """With each list build (per lane for debug stops and the lane mode, a
+    wave or a quad with the first lambda in a step)."""
     sc = scenes.dam_block()
     with Particles(sc.params) as c:
         c.load(sc.x, sc.v)
@@ -152,10 +158,10 @@ def test_list_capacity_at_and_below_the_largest_list():
     assert nmax > 8
     ref = run(sc.params, sc.x, sc.v, 1)
     # Capacity = max_neighbors rounded up to whole 4-entry chunks.
-    got = run(sc.params.replace(max_neighbors=nmax), sc.x, sc.v, 1)
+    got = run(sc.params.replace(max_neighbors=nmax), sc.x, sc.v, 1, mode)
     for a, b in zip(ref, got):
         assert bits_equal(a, b)
-    with Particles(sc.params.replace(max_neighbors=4 * ((nmax - 1) // 4))) as c:
+    with context(sc.params.replace(max_neighbors=4 * ((nmax - 1) // 4)), mode) as c:
         c.load(sc.x, sc.v)
         with pytest.raises(PbfError) as e:
             c.time_step()
