@@ -229,15 +229,14 @@ int finish_create(Ctx* x, pbf_ctx** out) {
   if (const char* w = getenv("PBF_WIDE_MAX"))
     for (pbf::PassRule& r : x->rules) r.wave_max = atoll(w);
   if (const char* w = getenv("PBF_QUAD_MAX"))
-    for (int k : {pbf::kPassBuild, pbf::kPassLambda, pbf::kPassDeltap, pbf::kPassXsph})
-      x->rules[k].quad_max = atoll(w);
+    for (pbf::PassRule& r : x->rules) r.quad_max = atoll(w);
   // PBF_RULE_<PASS>=wave_max,quad_max sets one pass's bounds.
   static const char* const kRuleEnv[pbf::kListPasses] = {
       "PBF_RULE_BUILD", "PBF_RULE_LAMBDA", "PBF_RULE_DELTAP", "PBF_RULE_XSPH", "PBF_RULE_VORT"};
   for (int k = 0; k < pbf::kListPasses; k++) {
     long long wm = 0, qm = 0;
     if (const char* w = getenv(kRuleEnv[k]))
-      if (sscanf(w, "%lld,%lld", &wm, &qm) == 2) x->rules[k] = {wm, k == pbf::kPassVort ? 0 : qm};
+      if (sscanf(w, "%lld,%lld", &wm, &qm) == 2) x->rules[k] = {wm, qm};
   }
   if (verify_math(x) != PBF_OK) {
     std::string m = x->err;

@@ -56,7 +56,8 @@ struct DevState {
 
 // Lanes per particle of each list pass, by state size: one wave (64 lanes)
 // for states of at most wave_max particles, one DPP quad (4 lanes; single
-// scenes; not the vorticity pass) up to quad_max, else one lane.
+// scenes; for the vorticity pass only its launch that does not carry the
+// next step's prediction) up to quad_max, else one lane.
 // Same bits in every case.  The bounds sit between the sizes where the
 // kernel time per launch crosses over (rocprofv3 kernel traces of lattice
 // blocks at 0.5 h spacing, 4 iterations; microseconds per launch, wave /
@@ -67,6 +68,8 @@ struct DevState {
 //   delta-p                8,192:  6.9 / 8.7 / 13.0   16,384: 10.1 / 8.9  / 12.9
 //   XSPH                  16,384:  8.2 / 9.3 / 14.1   24,576: 11.3 / 10.1 / 14.4
 //   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
+//   vorticity commit      32,768:  -   / 8.6 / 11.0   65,536:  -   / 10.2 / 11.1
+//                         98,304:  -   / 13.6 / 12.6
 // and quads against one lane at 98,304: lambda 13.6 / 14.4, delta-p 13.8 /
 // 15.0, XSPH 17.0 / 18.1; at 131,072 the build 56.3 / 55.0, and the step
 // slower with quads.  (The quad build's figure includes its separate first
@@ -83,7 +86,7 @@ constexpr PassRule kPassRules[kListPasses] = {
     {20480, 114688},  // lambda
     {12288, 114688},  // delta-p
     {20480, 114688},  // XSPH
-    {12288, 0},       // vorticity (+ commit, + prediction)
+    {12288, 81920},   // vorticity (+ commit, + prediction)
 };
 
 struct Ctx {

@@ -1352,6 +1352,19 @@ struct XVRec {
   float3 x, v;
 };
 
+// The vorticity confinement and commit (no next prediction) with a quad per
+// particle.
+__global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, const float4* in) {
+  const uint32_t i = quad_index();
+  if (i >= d.n) return;
+  const float4 xi = in[i];
+  float s[3];
+  quad_sums<3>(
+      i, (int)d.ncnt[i], d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
+      [&](uint32_t, const float4& xj, float* t) { vort_terms(c, xi, xj, t); }, s);
+  if ((threadIdx.x & 3u) == 0) vort_finish(i, d, c, xi, s[0], s[1], s[2], nullptr, nullptr, true);
+}
+
 __global__ __launch_bounds__(kBlock) void k_xsph_quad(Dev d, SimConst c, float4* out) {
   const uint32_t i = quad_index();
   if (i >= d.n) return;
@@ -1713,6 +1726,10 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
 }
 
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
+  if (quad(x, kPassVort)) {
+    k_vort_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, in);
+    return hipGetLastError();
+  }
   if (wide(x, kPassVort)) {
     k_vort_wide<false, false><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, x.grids[0], in);
     return hipGetLastError();
