@@ -112,6 +112,10 @@ void drop_graph(Ctx* x) {
   if (x->graph) (void)hipGraphDestroy(x->graph);
   x->graph_exec = nullptr;
   x->graph = nullptr;
+  if (x->graph1_exec) (void)hipGraphExecDestroy(x->graph1_exec);
+  if (x->graph1) (void)hipGraphDestroy(x->graph1);
+  x->graph1_exec = nullptr;
+  x->graph1 = nullptr;
 }
 
 template <class T>
@@ -226,6 +230,8 @@ int finish_create(Ctx* x, pbf_ctx** out) {
   // PBF_NO_SMALL_SORT keeps the multi-launch scan and sort for small states
   // too (same bits; for measurement).
   x->small_sort = getenv("PBF_NO_SMALL_SORT") == nullptr;
+  // PBF_STEP_GRAPH=0: one-step calls of the flat schedule keep plain launches.
+  if (const char* w = getenv("PBF_STEP_GRAPH")) x->step_graph = w[0] != '0';
   if (const char* w = getenv("PBF_WIDE_MAX"))
     for (pbf::PassRule& r : x->rules) r.wave_max = atoll(w);
   if (const char* w = getenv("PBF_QUAD_MAX"))
@@ -767,8 +773,37 @@ int enqueue_steps(Ctx* x, float dt, int nsteps) {
     for (int s = 1; s + 1 < nsteps; s++) HIPCHK(x, hipGraphLaunch(x->graph_exec, x->stream));
     rc = run_step(x, k, 0, nullptr, true, false);
     if (rc != PBF_OK) return rc;
+  } else if (nsteps == 1 &&
+             (x->schedule == PBF_SCHED_GRAPH ||
+              (x->schedule == PBF_SCHED_FLAT && x->step_graph && dt == x->last_dt1))) {
+    // A call of one step (the reference's timeStep per frame) replays one
+    // captured whole step, from the prediction to the plain commit: one
+    // launch from the host instead of one per pass.  The flat schedule does
+    // so once a second one-step call in a row has the same dt, so that a
+    // caller who changes dt every call is not made to capture every call.
+    x->last_dt1 = dt;
+    if (!x->graph1_exec || x->graph1_dt != dt) {
+      if (x->graph1_exec) (void)hipGraphExecDestroy(x->graph1_exec);
+      if (x->graph1) (void)hipGraphDestroy(x->graph1);
+      x->graph1_exec = nullptr;
+      x->graph1 = nullptr;
+      HIPCHK(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = run_step(x, k, 0, nullptr, false, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(x->stream, &g);
+      if (rc != PBF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(x, PBF_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+      x->graph1 = g;
+      HIPCHK(x, hipGraphInstantiate(&x->graph1_exec, g, nullptr, nullptr, 0));
+      x->graph1_dt = dt;
+    }
+    HIPCHK(x, hipGraphLaunch(x->graph1_exec, x->stream));
   } else {
-    // Flat schedule, and the graph schedule for calls of one or two steps.
+    // Flat schedule, and the graph schedule for calls of two steps.
+    if (nsteps == 1) x->last_dt1 = dt;
     for (int s = 0; s < nsteps; s++) {
       const int rc = run_step(x, k, 0, nullptr, s > 0, s + 1 < nsteps);
       if (rc != PBF_OK) return rc;

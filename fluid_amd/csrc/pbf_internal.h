@@ -135,6 +135,11 @@ struct Ctx {
   hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)
   hipGraphExec_t graph_exec = nullptr;
   float graph_dt = 0.0f;
+  hipGraph_t graph1 = nullptr;       // captured whole step, for calls of one step
+  hipGraphExec_t graph1_exec = nullptr;
+  float graph1_dt = 0.0f;
+  bool step_graph = true;            // flat schedule: replay graph1 when dt repeats
+  float last_dt1 = -1.0f;            // dt of the previous one-step call
   std::string err;
 };
 

@@ -375,8 +375,9 @@ def test_schedules_bit_exact_with_flat(schedule, block):
 
 
 def test_graph_replays_each_match_eager():
-    """Replay the captured inner step over several calls (calls of three or
-    more steps replay it, shorter calls run plain launches) with eager work
+    """Replay the captured steps over several calls (calls of three or more
+    steps replay the captured inner step, calls of one step a captured
+    whole step, calls of two steps run plain launches) with eager work
     (reads, synchronisations) in between; every call must equal the flat
     schedule's run."""
     p, spec, x0, v0 = _batched_items(4)
@@ -390,6 +391,41 @@ def test_graph_replays_each_match_eager():
             a, b = f.read(), g.read()
             for u, w in zip(a, b):
                 assert bits_equal(u, w)
+
+
+def test_graph_one_step_calls_match_flat(orc):
+    """The reference's use: one timeStep per frame.  One-step calls replay a
+    captured whole step with the graph schedule, and with the flat schedule
+    once dt repeats; twelve calls, a read after each, a change of dt
+    (recapture) after six and a three-step call between, all equal to plain
+    launches (a context created with PBF_STEP_GRAPH=0), and five one-step
+    calls equal the oracle's five steps."""
+    sc = scenes.two_spheres(per_sphere=3000, spacing=0.04, steps=5)
+    p = sc.params.replace(box_hi=(5.0, 3.0, 3.0))
+    os.environ["PBF_STEP_GRAPH"] = "0"
+    try:
+        plain = ctx_for(p, sc.x, sc.v)
+    finally:
+        os.environ.pop("PBF_STEP_GRAPH", None)
+    with plain as f, ctx_for(p, sc.x, sc.v) as g, ctx_for(p, sc.x, sc.v) as d:
+        g.set_schedule("graph")
+        for k in range(12):
+            dt = None if k < 6 else 0.5 * p.dt
+            n = 3 if k == 8 else 1
+            for c in (f, g, d):
+                c.time_step(dt=dt, nsteps=n)
+            a = f.read()
+            for c in (g, d):
+                for u, w in zip(a, c.read()):
+                    assert bits_equal(u, w)
+    for sched in ("flat", "graph"):
+        with ctx_for(p, sc.x, sc.v) as g:
+            g.set_schedule(sched)
+            for _ in range(5):
+                g.time_step()
+            gx, gv, _ = g.read()
+        xg, vg, _ = orc.step(p, sc.x, sc.v, 5, orc.NBR_GRID)
+        assert bits_equal(gx, xg) and bits_equal(gv, vg)
 
 
 def test_async_steps_and_overflow_reported_by_sync():
