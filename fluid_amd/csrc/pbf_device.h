@@ -678,16 +678,21 @@ __device__ __forceinline__ void for_each_neighbor_pipe(uint32_t i, int cnt, int 
 }
 
 // Four lanes per particle (one DPP quad).  Lane k = lane % 4 of particle
-// i's quad gathers entry 4c + k of every 4-entry chunk c and evaluates its
-// NS terms with term(j, data, t); then every lane of the quad adds the
-// chunk's four terms to sums[] in list order, taking its partners' terms
-// with DPP quad_perm broadcasts of lanes 0, 1, 2, 3.  Entries past the list
-// end in the last chunk (padding slots, which hold a particle index)
-// contribute +0: a sum that starts at +0 is never -0, and s + (+0) = s for
-// every other s, so the sums are those of the lane-per-particle loops bit
-// for bit.  A quad's lanes share i and cnt, so they run the loop in step.
-// The next chunk's gather and the one after's index load are in flight
-// while a chunk's terms are evaluated.
+// i's quad walks chunks k, k + 4, k + 8, ... of its list (4 entries each),
+// gathering and evaluating their NS terms with term(j, data, t); each round
+// the four lanes hold chunks 4r .. 4r + 3, and every lane of the quad adds
+// their sixteen terms to sums[] in list order (chunk 4r's four entries,
+// then chunk 4r + 1's, ...), taking its partners' terms with DPP quad_perm
+// broadcasts of lanes 0, 1, 2, 3.  Entries past the list end (the padding
+// slots of the last chunk, which hold a particle index, and the chunks past
+// the last, which read particle i) contribute +0: a sum that starts at +0
+// is never -0, and s + (+0) = s for every other s, so the sums are those of
+// the lane-per-particle loops bit for bit.  A quad's lanes share i and cnt,
+// so they run the loop in step.  A particle's list is walked as four
+// chunk streams at once: a list of m entries takes ceil(m/16) rounds of
+// dependent loads instead of ceil(m/4), which is what sets the walk of the
+// longest lists (the index load of a lane's next chunk is in flight while
+// a round's terms are evaluated).
 __device__ __forceinline__ float quad_lane(float t, int k) {
   int r;
   switch (k) {  // the DPP control must be a constant
@@ -708,39 +713,36 @@ __device__ __forceinline__ void quad_sums(uint32_t i, int cnt, int K4,
 #pragma unroll
   for (int q = 0; q < NS; q++) sums[q] = 0.0f;
   auto idx = [&](int ch) { return load_chunk(nbr + nbr_chunk(i, ch, K4)); };
-  auto pick = [&](const uint4& e) { return k == 0 ? e.x : k == 1 ? e.y : k == 2 ? e.z : e.w; };
   const int nch = (cnt + 3) >> 2;
-  uint32_t j = i, jn = i;
-  uint4 an = make_uint4(i, i, i, i);
-  D dj{};
-  if (nch > 0) {
-    j = pick(idx(0));
-    dj = load(j);
-  }
-  if (nch > 1) an = idx(1);
-  for (int ch = 0; ch < nch; ch++) {
-    D dn{};
-    uint4 ann = an;
-    if (ch + 1 < nch) {
-      jn = pick(an);
-      dn = load(jn);
-    }
-    if (ch + 2 < nch) ann = idx(ch + 2);
-    float t[NS];
-    term(j, dj, t);
-    if (4 * ch + k >= cnt) {
+  const int rounds = (nch + 3) >> 2;
+  uint4 e = make_uint4(i, i, i, i);
+  if (k < nch) e = idx(k);
+  for (int r = 0; r < rounds; r++) {
+    const int ch = 4 * r + k;
+    const D d0 = load(e.x), d1 = load(e.y), d2 = load(e.z), d3 = load(e.w);
+    uint4 en = make_uint4(i, i, i, i);
+    if (ch + 4 < nch) en = idx(ch + 4);
+    float t[4][NS];
+    term(e.x, d0, t[0]);
+    term(e.y, d1, t[1]);
+    term(e.z, d2, t[2]);
+    term(e.w, d3, t[3]);
 #pragma unroll
-      for (int q = 0; q < NS; q++) t[q] = 0.0f;
-    }
-    // Entries 4ch + 0 .. 3, in order.
+    for (int u = 0; u < 4; u++) {
+      if (4 * ch + u >= cnt) {
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-#pragma unroll
-      for (int q = 0; q < NS; q++) sums[q] = sums[q] + quad_lane(t[q], s);
+        for (int q = 0; q < NS; q++) t[u][q] = 0.0f;
+      }
     }
-    j = jn;
-    dj = dn;
-    an = ann;
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+#pragma unroll
+        for (int q = 0; q < NS; q++) sums[q] = sums[q] + quad_lane(t[u][q], kk);
+      }
+    }
+    e = en;
   }
 }
 

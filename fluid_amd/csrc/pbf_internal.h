@@ -52,6 +52,7 @@ struct DevState {
   float4* meshTri = nullptr;      // collider triangles, 6 float4 each (pbf_device.h MeshView)
   float4* meshNode = nullptr;     // collider BVH, 2 float4 per node
   uint8_t* meshFlag = nullptr;    // scene-grid cells near a BVH leaf box (MeshView::flag)
+  uint32_t* dense = nullptr;      // particles with long lists, queued by the build (n)
 };
 
 // Lanes per particle of each list pass, by state size: one wave (64 lanes)
@@ -61,19 +62,20 @@ struct DevState {
 // Same bits in every case.  The bounds sit between the sizes where the
 // kernel time per launch crosses over (rocprofv3 kernel traces of lattice
 // blocks at 0.5 h spacing, 4 iterations; microseconds per launch, wave /
-// quad / lane):
-//   build + first lambda  16,384: 20.7 / 26.0 / -     24,576: 27.3 / 26.9 / -
-//                         32,768: 34.3 / 27.3 / 44.9  98,304: -    / 43.4 / 52.1
-//   lambda                16,384:  7.3 / 8.3 / 12.3   24,576:  9.8 / 9.0  / 12.5
-//   delta-p                8,192:  6.9 / 8.7 / 13.0   16,384: 10.1 / 8.9  / 12.9
-//   XSPH                  16,384:  8.2 / 9.3 / 14.1   24,576: 11.3 / 10.1 / 14.4
+// quad / lane; the quad build's figure includes its separate first lambda
+// launch):
+//   build + first lambda   8,192: 13.6 / 22.6 / 42.6  16,384: 20.3 / 23.6 / 43.6
+//                         24,576: 27.2 / 25.4 / 44.5  65,536: -    / 32.7 / 46.9
+//                        131,072: -    / 55.6 / 54.4
+//   lambda                 8,192:  5.1 / 6.7 / 12.5   16,384:  7.3 / 6.8 / 12.5
+//                         65,536:  -   / 10.3 / 13.0 131,072:  -   / 16.1 / 15.0
+//   delta-p                8,192:  6.7 / 6.9 / 13.0   16,384:  9.9 / 6.9 / 12.9
+//                         65,536:  -   / 10.6 / 13.8 131,072:  -   / 17.3 / 15.4
+//   XSPH                   8,192:  5.5 / 7.8 / 13.9   16,384:  8.3 / 7.7 / 13.9
+//                         65,536:  -   / 12.6 / 15.2 131,072:  -   / 21.2 / 19.1
 //   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
 //   vorticity commit      32,768:  -   / 8.6 / 11.0   65,536:  -   / 10.2 / 11.1
-//                         98,304:  -   / 13.6 / 12.6
-// and quads against one lane at 98,304: lambda 13.6 / 14.4, delta-p 13.8 /
-// 15.0, XSPH 17.0 / 18.1; at 131,072 the build 56.3 / 55.0, and the step
-// slower with quads.  (The quad build's figure includes its separate first
-// lambda launch.)  PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX
+//                         98,304:  -   / 13.6 / 12.6  PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX
 // (quad_max) and PBF_RULE_<PASS>=wave_max,quad_max override them at context
 // creation; 0 turns a variant off.
 enum ListPass : int { kPassBuild, kPassLambda, kPassDeltap, kPassXsph, kPassVort, kListPasses };
@@ -82,12 +84,30 @@ struct PassRule {
   int64_t quad_max;
 };
 constexpr PassRule kPassRules[kListPasses] = {
-    {24576, 114688},  // build + first lambda
-    {20480, 114688},  // lambda
-    {12288, 114688},  // delta-p
-    {20480, 114688},  // XSPH
+    {20480, 114688},  // build + first lambda
+    {12288, 114688},  // lambda
+    {8192, 114688},   // delta-p
+    {12288, 114688},  // XSPH
     {12288, 81920},   // vorticity (+ commit, + prediction)
 };
+
+// Dense split: in single-scene states of at most kDenseMaxN particles whose
+// list build runs one lane per particle, the build queues the particles with more than dense_list
+// neighbours (count in flags[kDenseSlot]), and the per-lane lambda, delta-p
+// and XSPH launches leave them to quad workgroups appended to the same
+// launch.  A quad walks a list as four chunk streams at once (quad_sums), so
+// a compressed region's longest lists take a quarter of the dependent
+// rounds of the lone waves that otherwise set these launches' length; the
+// other particles keep one lane each.  Same bits (the quad sums are the
+// per-lane sums).  PBF_DENSE_LIST overrides the threshold (0: off).
+// Measured on the 262,144-particle dam break (200 steps): 1.32 ms per step
+// without, 0.97 at 128 (0.99 at 64, 1.07 at 256); uniform blocks of
+// 131,072 and 262,144 particles, with no list above 128, 1-1.6% slower for
+// the second half of workgroups, and the 4,096,000-particle run 0.6%
+// slower, hence the bound on n.
+constexpr uint32_t kDenseList = 128;
+constexpr int64_t kDenseMaxN = int64_t{1} << 20;
+constexpr int kDenseSlot = 12;
 
 struct Ctx {
   pbf_params p{};
@@ -124,6 +144,7 @@ struct Ctx {
   bool small_sort = true;            // single-workgroup scan + sort for small states
   PassRule rules[kListPasses] = {kPassRules[0], kPassRules[1], kPassRules[2], kPassRules[3],
                                  kPassRules[4]};
+  uint32_t dense_list = kDenseList;  // dense split threshold (0: off)
   // Density-field output on the device, kept between calls, and two pinned
   // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;

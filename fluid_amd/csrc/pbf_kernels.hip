@@ -51,9 +51,24 @@ struct Dev {
   const SceneGrid* grids;
   const uint32_t* slotScene;
   MeshView mesh;  // triangle-mesh collider (ntri 0: the analytic box walls)
+  // Dense split (kDenseList in pbf_internal.h): the queue (NULL: off), its
+  // threshold, and the per-lane workgroups of a split launch (0: not split).
+  uint32_t* dense;
+  uint32_t dense_list;
+  uint32_t nlane;
   uint32_t n;
   int K4;
 };
+
+inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Whether x queues its dense particles: a single scene of at most
+// kDenseMaxN particles whose list build runs one lane per particle.
+static bool dense_split(const Ctx& x) {
+  const PassRule& b = x.rules[kPassBuild];
+  const bool lane_build = x.n > b.wave_max && !(x.n <= b.quad_max && x.n < (int64_t{1} << 30));
+  return x.nscenes == 1 && x.d.dense && x.dense_list > 0 && x.n <= kDenseMaxN && lane_build;
+}
 
 Dev dev_of(const Ctx& x) {
   const DevState& s = x.d;
@@ -71,6 +86,9 @@ Dev dev_of(const Ctx& x) {
   d.mesh.skin = x.mesh_skin;
   d.mesh.flag = s.meshFlag;
   d.mesh.margin = x.mesh_margin;
+  d.dense = dense_split(x) ? s.dense : nullptr;
+  d.dense_list = x.dense_list;
+  d.nlane = 0;
   d.n = (uint32_t)x.n;
   d.K4 = k4_of(x);
   return d;
@@ -746,6 +764,84 @@ __device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimC
   out[id_of(d.posB[i])] = density_point(make_float3(xi.x, xi.y, xi.z), d, c, g);
 }
 
+// ----------------------------------------- one particle on one quad --
+// The lambda, delta-p and XSPH passes of particle i on the four lanes of a
+// quad (quad_sums); every lane of the quad calls them.
+__device__ __forceinline__ uint32_t quad_index() {
+  return (xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) >> 2;
+}
+
+template <bool PACKED>
+__device__ __forceinline__ void lambda_quad_one(uint32_t i, const Dev& d, const SimConst& c,
+                                                const float4* in, float4* out) {
+  const float3 xi = PACKED ? d.xs3[i] : make_float3(in[i].x, in[i].y, in[i].z);
+  float s[5];
+  quad_sums<5>(
+      i, (int)d.ncnt[i], d.K4, d.nbr,
+      [&](uint32_t j) {
+        if (PACKED) return at(d.xs3, j);
+        const float4 q = at(in, j);
+        return make_float3(q.x, q.y, q.z);
+      },
+      [&](uint32_t, const float3& xj, float* t) { lambda_terms(c, xi, xj, t); }, s);
+  if ((threadIdx.x & 3u) == 0)
+    out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, s[0], s[4], s[1], s[2], s[3]));
+}
+
+template <bool LAST, int NT>
+__device__ __forceinline__ void deltap_quad_one(uint32_t i, const Dev& d, const SimConst& c,
+                                                const SceneGrid& g, const float4* in) {
+  const float4 xi = in[i];
+  float s[3];
+  quad_sums<3>(
+      i, (int)d.ncnt[i], d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
+      [&](uint32_t, const float4& xj, float* t) { deltap_terms<NT>(c, xi, xj, t); }, s);
+  if ((threadIdx.x & 3u) == 0) deltap_finish<LAST>(i, d, c, g, xi, s[0], s[1], s[2]);
+}
+
+struct XVRec {
+  float3 x, v;
+};
+
+__device__ __forceinline__ void xsph_quad_one(uint32_t i, const Dev& d, const SimConst& c,
+                                              float4* out) {
+  const float3* __restrict__ xv = reinterpret_cast<const float3*>(d.xv);
+  const float3 xi = xv[2 * (size_t)i], vi = xv[2 * (size_t)i + 1];
+  float s[7];
+  quad_sums<7>(
+      i, (int)d.ncnt[i], d.K4, d.nbr,
+      [&](uint32_t j) { return XVRec{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
+      [&](uint32_t, const XVRec& q, float* t) { xsph_terms(c, xi, vi, q.x, q.v, t); }, s);
+  if ((threadIdx.x & 3u) == 0)
+    xsph_finish(i, d, c, xi, vi, s[0], s[1], s[2], s[3], s[4], s[5], s[6], out);
+}
+
+// Dense split.  In a split launch the first d.nlane workgroups run the
+// per-lane pass (lane_particle: ~0 for queued particles, which they skip)
+// and the rest take the queued particles, a quad each, in queue order (a
+// particle's result does not depend on which workgroup computes it).
+__device__ __forceinline__ bool dense_quads_part(const Dev& d) {
+  return d.nlane && blockIdx.x >= d.nlane;
+}
+__device__ __forceinline__ uint32_t lane_particle(const Dev& d) {
+  if (!d.nlane) return xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const uint32_t i = xcd_block(blockIdx.x, d.nlane) * kBlock + threadIdx.x;
+  return i < d.n && d.ncnt[i] > d.dense_list ? 0xffffffffu : i;
+}
+template <class F>
+__device__ __forceinline__ void for_dense_quads(const Dev& d, F f) {
+  const uint32_t qn = d.flags[kDenseSlot];
+  const uint32_t stride = (gridDim.x - d.nlane) * (kBlock / 4);
+  for (uint32_t q = (blockIdx.x - d.nlane) * (kBlock / 4) + (threadIdx.x >> 2); q < qn;
+       q += stride)
+    f(d.dense[q]);  // a quad's lanes share q
+}
+
+// Queues particle i (its list longer than the threshold) for the dense split.
+__device__ __forceinline__ void queue_dense(const Dev& d, uint32_t i, uint32_t cnt) {
+  if (cnt > d.dense_list) d.dense[atomicAdd(&d.flags[kDenseSlot], 1u)] = i;
+}
+
 // ------------------------------------------------------- flat schedule --
 template <bool MULTI>
 __device__ __forceinline__ SceneGrid grid_of(uint32_t i, const SceneGrid& g0, const Dev& d) {
@@ -797,7 +893,9 @@ __global__ __launch_bounds__(kBlock) void k_rank(Dev d, const float4* pred) {
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_nbr_build(Dev d, float h2, SceneGrid g0) {
   const uint32_t i = gather_index();
-  if (i < d.n) nbr_one(i, d, h2, grid_of<MULTI>(i, g0, d));
+  if (i >= d.n) return;
+  nbr_one(i, d, h2, grid_of<MULTI>(i, g0, d));
+  if (!MULTI && d.dense) queue_dense(d, i, d.ncnt[i]);
 }
 
 // The neighbour build followed, in the same thread, by the first density/λ
@@ -811,24 +909,37 @@ __global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneG
   if (i >= d.n) return;
   nbr_one(i, d, c.h2, grid_of<MULTI>(i, g0, d));
   lambda_one<false>(i, d, c, d.predQ, out);
+  if (!MULTI && d.dense) queue_dense(d, i, d.ncnt[i]);
 }
 
 template <bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_lambda(Dev d, SimConst c, const float4* in,
                                                    float4* out) {
-  const uint32_t i = gather_index();
+  if (dense_quads_part(d)) {
+    for_dense_quads(d, [&](uint32_t i) { lambda_quad_one<PACKED>(i, d, c, in, out); });
+    return;
+  }
+  const uint32_t i = lane_particle(d);
   if (i < d.n) lambda_one<PACKED>(i, d, c, in, out);
 }
 
 template <bool MULTI, bool LAST, int NT>
 __global__ __launch_bounds__(kBlock) void k_deltap(Dev d, SimConst c, SceneGrid g0,
                                                    const float4* in, float4* out) {
-  const uint32_t i = gather_index();
+  if (!MULTI && dense_quads_part(d)) {
+    for_dense_quads(d, [&](uint32_t i) { deltap_quad_one<LAST, NT>(i, d, c, g0, in); });
+    return;
+  }
+  const uint32_t i = lane_particle(d);
   if (i < d.n) deltap_one<LAST, NT>(i, d, c, grid_of<MULTI>(i, g0, d), in, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_xsph(Dev d, SimConst c, float4* out) {
-  const uint32_t i = gather_index();
+  if (dense_quads_part(d)) {
+    for_dense_quads(d, [&](uint32_t i) { xsph_quad_one(i, d, c, out); });
+    return;
+  }
+  const uint32_t i = lane_particle(d);
   if (i < d.n) xsph_one(i, d, c, out);
 }
 
@@ -1230,40 +1341,18 @@ __global__ __launch_bounds__(kWide) void k_vort_wide(Dev d, SimConst c, SceneGri
 // costs a sequential sum in one lane.  With a quad per particle (quad_sums)
 // there are four times the waves of the lane-per-particle passes and each
 // walks a quarter of the list; the sums are the same bits.
-__device__ __forceinline__ uint32_t quad_index() {
-  return (xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) >> 2;
-}
-
 template <bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_lambda_quad(Dev d, SimConst c, const float4* in,
                                                         float4* out) {
   const uint32_t i = quad_index();
-  if (i >= d.n) return;  // a quad's lanes leave together
-  const float3 xi = PACKED ? d.xs3[i] : make_float3(in[i].x, in[i].y, in[i].z);
-  float s[5];
-  quad_sums<5>(
-      i, (int)d.ncnt[i], d.K4, d.nbr,
-      [&](uint32_t j) {
-        if (PACKED) return at(d.xs3, j);
-        const float4 q = at(in, j);
-        return make_float3(q.x, q.y, q.z);
-      },
-      [&](uint32_t, const float3& xj, float* t) { lambda_terms(c, xi, xj, t); }, s);
-  if ((threadIdx.x & 3u) == 0)
-    out[i] = make_float4(xi.x, xi.y, xi.z, lambda_of(c, s[0], s[4], s[1], s[2], s[3]));
+  if (i < d.n) lambda_quad_one<PACKED>(i, d, c, in, out);  // a quad's lanes leave together
 }
 
 template <bool LAST, int NT>
 __global__ __launch_bounds__(kBlock) void k_deltap_quad(Dev d, SimConst c, SceneGrid g,
                                                         const float4* in) {
   const uint32_t i = quad_index();
-  if (i >= d.n) return;
-  const float4 xi = in[i];
-  float s[3];
-  quad_sums<3>(
-      i, (int)d.ncnt[i], d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
-      [&](uint32_t, const float4& xj, float* t) { deltap_terms<NT>(c, xi, xj, t); }, s);
-  if ((threadIdx.x & 3u) == 0) deltap_finish<LAST>(i, d, c, g, xi, s[0], s[1], s[2]);
+  if (i < d.n) deltap_quad_one<LAST, NT>(i, d, c, g, in);
 }
 
 // The neighbour build with a quad per particle (single scene; nbr_one's
@@ -1348,10 +1437,6 @@ __global__ __launch_bounds__(kBlock) void k_nbr_quad(Dev d, float h2, SceneGrid 
   }
 }
 
-struct XVRec {
-  float3 x, v;
-};
-
 // The vorticity confinement and commit (no next prediction) with a quad per
 // particle.
 __global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, const float4* in) {
@@ -1367,16 +1452,7 @@ __global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, const f
 
 __global__ __launch_bounds__(kBlock) void k_xsph_quad(Dev d, SimConst c, float4* out) {
   const uint32_t i = quad_index();
-  if (i >= d.n) return;
-  const float3* __restrict__ xv = reinterpret_cast<const float3*>(d.xv);
-  const float3 xi = xv[2 * (size_t)i], vi = xv[2 * (size_t)i + 1];
-  float s[7];
-  quad_sums<7>(
-      i, (int)d.ncnt[i], d.K4, d.nbr,
-      [&](uint32_t j) { return XVRec{at(xv, 2 * j), at(xv, 2 * j + 1)}; },
-      [&](uint32_t, const XVRec& q, float* t) { xsph_terms(c, xi, vi, q.x, q.v, t); }, s);
-  if ((threadIdx.x & 3u) == 0)
-    xsph_finish(i, d, c, xi, vi, s[0], s[1], s[2], s[3], s[4], s[5], s[6], out);
+  if (i < d.n) xsph_quad_one(i, d, c, out);
 }
 
 // Scan, scatter and rank of a small state in one workgroup: the three
@@ -1484,7 +1560,6 @@ __global__ __launch_bounds__(kBlock) void k_verify_math(float b, float rb, int d
   if (ns) atomicAdd(bad + 2, ns);
 }
 
-inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
@@ -1560,8 +1635,26 @@ static bool quad(const Ctx& x, ListPass p) { return lanes(x, p) == 4; }
 static unsigned wide_grid(const Ctx& x) { return (unsigned)x.n; }
 static unsigned quad_grid(const Ctx& x) { return blocks_for(x.n * 4); }
 
+// The dense queue starts empty at every build.
+static hipError_t clear_dense(const Ctx& x, const Dev& d) {
+  return d.dense ? launch_fill_u32(x.d.flags + kDenseSlot, 1, 0u, x.stream) : hipSuccess;
+}
+// A per-lane pass over the particles, split with the dense queue when x
+// has one: the per-lane workgroups, then as many quad workgroups.
+static Dev split_dev(const Ctx& x, ListPass p, unsigned* grid) {
+  Dev d = dev_of(x);
+  *grid = blocks_for(x.n);
+  if (d.dense && lanes(x, p) == 1) {
+    d.nlane = *grid;
+    *grid *= 2u;
+  }
+  return d;
+}
+
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
   const Dev d = dev_of(x);
+  const hipError_t e = clear_dense(x, d);
+  if (e != hipSuccess) return e;
   if (quad(x, kPassBuild)) {
     k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
     return hipGetLastError();
@@ -1575,6 +1668,8 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
 
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   const Dev d = dev_of(x);
+  const hipError_t e = clear_dense(x, d);
+  if (e != hipSuccess) return e;
   if (quad(x, kPassBuild)) {
     // The quad build, then the first lambda pass over the finished lists.
     k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
@@ -1610,10 +1705,12 @@ hipError_t launch_lambda(Ctx& x, const SimConst& c, const float4* in, float4* ou
       k_lambda_wide<false><<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, in, out);
     return hipGetLastError();
   }
+  unsigned ng = 0;
+  const Dev d = split_dev(x, kPassLambda, &ng);
   if (packed)
-    k_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+    k_lambda<true><<<ng, kBlock, 0, x.stream>>>(d, c, in, out);
   else
-    k_lambda<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, in, out);
+    k_lambda<false><<<ng, kBlock, 0, x.stream>>>(d, c, in, out);
   return hipGetLastError();
 }
 
@@ -1649,10 +1746,12 @@ static void launch_deltap_nt(const Ctx& x, const Dev& d, const SimConst& c, cons
     else
       k_deltap<true, false, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
   } else {
+    unsigned ng = 0;
+    const Dev ds = split_dev(x, kPassDeltap, &ng);
     if (last)
-      k_deltap<false, true, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<false, true, NT><<<ng, kBlock, 0, x.stream>>>(ds, c, x.grids[0], in, out);
     else
-      k_deltap<false, false, NT><<<nb, kBlock, 0, x.stream>>>(d, c, x.grids[0], in, out);
+      k_deltap<false, false, NT><<<ng, kBlock, 0, x.stream>>>(ds, c, x.grids[0], in, out);
   }
 }
 
@@ -1674,7 +1773,9 @@ hipError_t launch_xsph(Ctx& x, const SimConst& c, float4* out) {
     k_xsph_wide<<<wide_grid(x), kWide, 0, x.stream>>>(dev_of(x), c, out);
     return hipGetLastError();
   }
-  k_xsph<<<blocks_for(x.n), kBlock, 0, x.stream>>>(dev_of(x), c, out);
+  unsigned ng = 0;
+  const Dev d = split_dev(x, kPassXsph, &ng);
+  k_xsph<<<ng, kBlock, 0, x.stream>>>(d, c, out);
   return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@ r = h (the neighbour test is inclusive, :258-265), particle counts around
 the wave and tile sizes (1 .. 257), particles starting on the walls, and
 the neighbour-list capacity at and one chunk below the largest list.  The
 step cases run with each variant of the list passes: a wave, a quad (4
-lanes) or a lane per particle."""
+lanes) or a lane per particle, and a lane with the long lists split off to
+quads."""
 import os
 
 import numpy as np
@@ -26,12 +27,15 @@ def bits_equal(a, b):
 
 
 # Context environments that select the list-pass variant for these small
-# states: the default rule (a wave per particle), quads for lambda, delta-p
-# and XSPH (lanes for the build and vorticity), one lane throughout.
+# states: the default rule (a wave per particle), quads (lanes for the
+# vorticity pass that carries the next prediction), one lane throughout,
+# and one lane with the dense split.
 MODES = {
     "wave": {},
     "quad": {"PBF_WIDE_MAX": "0"},
-    "lane": {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0"},
+    "lane": {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "0"},
+    # one lane per particle, with every list longer than 8 split off to quads
+    "split": {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "8"},
 }
 
 

@@ -227,7 +227,7 @@ def test_small_state_single_workgroup_sort_over_several_scan_chunks(orc):
 LANE_MODES = (
     {},  # the default rule: a wave per particle for small states
     {"PBF_WIDE_MAX": "0"},  # quads for lambda / delta-p / XSPH, lanes for the rest
-    {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0"},  # one lane per particle throughout
+    {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "0"},  # one lane throughout
 )
 
 
@@ -266,6 +266,28 @@ def test_wide_quad_and_lane_passes_equal(orc, spacing_over_h):
         assert outs[0][1][1] > 64  # several 64-entry blocks per list
     for (gx, gv, gr), _ in outs:
         assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
+
+
+@pytest.mark.parametrize("dense_list", ["60", "120", "0"])
+def test_dense_split_bit_exact(orc, dense_list):
+    """One lane per particle with the dense split: the build queues the
+    particles with more than PBF_DENSE_LIST neighbours (a block at spacing
+    0.3 h holds about 150 at first and 98 at most after three steps, so 60
+    queues many particles at every step, 120 only in the first steps; 0
+    turns the split off) and the lambda, delta-p and XSPH launches give them
+    to quad workgroups.  Three steps equal the oracle's."""
+    sc = scenes.dam_block()
+    h = sc.params.h
+    s = 0.3 * h
+    x = scenes.lattice_block(16, 16, 16, s, (0.1, 0.1, 0.1)).astype(np.float32)
+    x += np.random.default_rng(4).uniform(-1e-3 * s, 1e-3 * s, x.shape).astype(np.float32)
+    v = np.random.default_rng(5).uniform(-0.2, 0.2, x.shape).astype(np.float32)
+    p = sc.params.replace(max_neighbors=512)
+    xg, vg, rg = orc.step(p, x, v, 3, orc.NBR_GRID)
+    env = {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": dense_list}
+    ((gx, gv, gr), (nbar, nmax)), = run_modes(p, x, v, 3, (env,))
+    assert nmax > 60
+    assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
 
 
 def test_mid_size_state_quad_passes_bit_exact(orc):
