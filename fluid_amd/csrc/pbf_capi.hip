@@ -102,6 +102,7 @@ void free_device(Ctx* x) {
   dfree(d.cellCount); dfree(d.cellStart); dfree(d.blockSums);
   dfree(d.nbr); dfree(d.ncnt); dfree(d.flags);
   dfree(d.grids); dfree(d.slotScene); dfree(d.runs); dfree(d.stage); dfree(d.dense);
+  dfree(d.dense_b);
   x->alloc_n = 0;
   x->alloc_cells = 0;
 }
@@ -142,6 +143,7 @@ int ensure_device(Ctx* x) {
   DA(d.grids, (size_t)x->nscenes); DA(d.slotScene, np); DA(d.runs, (size_t)x->nscenes);
   DA(d.stage, 4 * np);
   DA(d.dense, np);
+  DA(d.dense_b, np);
 #undef DA
   if (e != hipSuccess) {
     free_device(x);
@@ -238,6 +240,7 @@ int finish_create(Ctx* x, pbf_ctx** out) {
   if (const char* w = getenv("PBF_QUAD_MAX"))
     for (pbf::PassRule& r : x->rules) r.quad_max = atoll(w);
   if (const char* w = getenv("PBF_DENSE_LIST")) x->dense_list = (uint32_t)atoll(w);
+  if (const char* w = getenv("PBF_DENSE_CELL")) x->dense_cell = (uint32_t)atoll(w);
   // PBF_RULE_<PASS>=wave_max,quad_max sets one pass's bounds.
   static const char* const kRuleEnv[pbf::kListPasses] = {
       "PBF_RULE_BUILD", "PBF_RULE_LAMBDA", "PBF_RULE_DELTAP", "PBF_RULE_XSPH", "PBF_RULE_VORT"};
@@ -1352,6 +1355,7 @@ size_t carve_state(unsigned char* base, size_t n, size_t ncell, size_t K, pbf::D
   take(d->grids, 1); take(d->slotScene, np); take(d->runs, 1);
   d->stage = nullptr;
   d->dense = nullptr;  // no dense split here
+  d->dense_b = nullptr;
   return off;
 }
 

@@ -53,6 +53,7 @@ struct DevState {
   float4* meshNode = nullptr;     // collider BVH, 2 float4 per node
   uint8_t* meshFlag = nullptr;    // scene-grid cells near a BVH leaf box (MeshView::flag)
   uint32_t* dense = nullptr;      // particles with long lists, queued by the build (n)
+  uint32_t* dense_b = nullptr;    // particles in crowded cells, queued by the sort (n)
 };
 
 // Lanes per particle of each list pass, by state size: one wave (64 lanes)
@@ -108,6 +109,14 @@ constexpr PassRule kPassRules[kListPasses] = {
 constexpr uint32_t kDenseList = 128;
 constexpr int64_t kDenseMaxN = int64_t{1} << 20;
 constexpr int kDenseSlot = 12;
+// The list build is split the same way, by what is known before it: the
+// sort queues the particles whose own cell holds more than dense_cell
+// particles (count in flags[kDenseBuildSlot]), and quad workgroups appended
+// to the per-lane build + first lambda launch build their lists with a quad
+// each (k_nbr_quad's walk) and run their first lambda on the same quad.
+// PBF_DENSE_CELL overrides the threshold (0: off).
+constexpr uint32_t kDenseCell = 32;
+constexpr int kDenseBuildSlot = 13;
 
 struct Ctx {
   pbf_params p{};
@@ -145,6 +154,7 @@ struct Ctx {
   PassRule rules[kListPasses] = {kPassRules[0], kPassRules[1], kPassRules[2], kPassRules[3],
                                  kPassRules[4]};
   uint32_t dense_list = kDenseList;  // dense split threshold (0: off)
+  uint32_t dense_cell = kDenseCell;  // build split threshold (0: off)
   // Density-field output on the device, kept between calls, and two pinned
   // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;

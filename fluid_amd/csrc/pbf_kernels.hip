@@ -56,6 +56,10 @@ struct Dev {
   uint32_t* dense;
   uint32_t dense_list;
   uint32_t nlane;
+  // Build split: the queue of particles in crowded cells (NULL: off) and its
+  // threshold on the cell's population.
+  uint32_t* dense_b;
+  uint32_t dense_cell;
   uint32_t n;
   int K4;
 };
@@ -88,6 +92,8 @@ Dev dev_of(const Ctx& x) {
   d.mesh.margin = x.mesh_margin;
   d.dense = dense_split(x) ? s.dense : nullptr;
   d.dense_list = x.dense_list;
+  d.dense_b = dense_split(x) && x.dense_cell > 0 ? s.dense_b : nullptr;
+  d.dense_cell = x.dense_cell;
   d.nlane = 0;
   d.n = (uint32_t)x.n;
   d.K4 = k4_of(x);
@@ -351,6 +357,11 @@ __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d,
   d.velB[dst] = v;
   d.predQ[dst] = q;
   d.cellSorted[dst] = c;
+  // Build split: a particle of a crowded cell gets its list from a quad.
+  if (d.dense_b && e - s > d.dense_cell) {
+    const uint32_t slot = atomicAdd(&d.flags[kDenseBuildSlot], 1u);
+    if (slot < d.n) d.dense_b[slot] = dst;  // each particle once per step: slot < n
+  }
 }
 
 // Distance from coordinate x to the near face of cell k+side of the axis
@@ -816,6 +827,87 @@ __device__ __forceinline__ void xsph_quad_one(uint32_t i, const Dev& d, const Si
     xsph_finish(i, d, c, xi, vi, s[0], s[1], s[2], s[3], s[4], s[5], s[6], out);
 }
 
+// The neighbour build with a quad per particle (single scene; nbr_one's
+// rows, pruning and order).  The quad walks each row 16 candidates at a
+// time: lane k tests candidates k, k + 4, k + 8, k + 12, and four DPP
+// broadcasts give every lane the quad's 16-bit hit mask in candidate order.
+// A hit lane stores its entry at slot cnt + (hits before it in the mask),
+// so the list is nbr_one's.  The slots past the end of the last chunk hold
+// i (a particle index, as the readers' gathers need; their terms are not
+// added).  The first lambda pass follows as its own quad launch.
+__device__ __forceinline__ uint32_t quad_or(uint32_t m) {
+  return (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 0)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 1)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 2)) |
+         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 3));
+}
+
+__device__ __forceinline__ void nbr_quad_one(uint32_t i, const Dev& d, float h2,
+                                             const SceneGrid& g) {
+  const uint32_t k = threadIdx.x & 3u;
+  const float4 xi = d.predQ[i];
+  int cc[3];
+  cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
+  const float cell = 1.0f / g.inv_cell;
+  const float hm2 = h2 * 1.0201f;  // (1.01 h)^2
+  const float4* __restrict__ pred = d.predQ;
+  const uint32_t cap = (uint32_t)d.K4 * kChunk;
+  auto hit = [&](const float4& xj) -> uint32_t {
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    return d2 <= h2 ? 1u : 0u;
+  };
+  auto put = [&](uint32_t slot, uint32_t j) {
+    if (slot < cap) d.nbr[nbr_chunk(i, (int)(slot >> 2), d.K4) + (slot & 3u)] = j;
+  };
+  uint32_t cnt = 0;
+  const float gxl = face_gap(xi.x, g.lo[0], cell, cc[0], -1);
+  const float gxr = face_gap(xi.x, g.lo[0], cell, cc[0], 1);
+  for (int dz = -1; dz <= 1; dz++) {
+    const int z = cc[2] + dz;
+    if (z < 0 || z >= g.dims[2]) continue;
+    const float gz = face_gap(xi.z, g.lo[2], cell, cc[2], dz);
+    for (int dy = -1; dy <= 1; dy++) {
+      const int y = cc[1] + dy;
+      if (y < 0 || y >= g.dims[1]) continue;
+      const float gy = face_gap(xi.y, g.lo[1], cell, cc[1], dy);
+      const float yz2 = gy * gy + gz * gz;
+      if (yz2 > hm2) continue;
+      const int x0 = (cc[0] > 0 && gxl * gxl + yz2 <= hm2) ? cc[0] - 1 : cc[0];
+      const int x1 = (cc[0] + 1 < g.dims[0] && gxr * gxr + yz2 <= hm2) ? cc[0] + 1 : cc[0];
+      const uint32_t row =
+          g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
+      const uint32_t s = d.cellStart[row + (uint32_t)x0];
+      const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
+      for (uint32_t jb = s; jb < e; jb += 16) {
+        // Up to 15 candidates past the range are read (the x* arrays are
+        // padded by 64 entries); their bits are dropped.
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          const uint32_t j = jb + k + 4u * u;
+          mine |= (hit(at(pred, j)) & (uint32_t)(j < e && j != i)) << (4u * u + k);
+        }
+        const uint32_t m = quad_or(mine);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          const uint32_t t = 4u * u + k;
+          if ((m >> t) & 1u) put(cnt + (uint32_t)__builtin_popcount(m & ((1u << t) - 1u)), jb + t);
+        }
+        cnt += (uint32_t)__builtin_popcount(m);
+      }
+    }
+  }
+  // Slots cnt .. end of its chunk: lane k fills slot cnt + k if that is in
+  // the same chunk.
+  const uint32_t pad = cnt + k;
+  if ((cnt & 3u) && (pad >> 2) == (cnt >> 2)) put(pad, i);
+  if (k == 0) {
+    if (cnt > cap) atomicMax(&d.flags[0], cnt);
+    d.ncnt[i] = min(cnt, cap);
+  }
+}
+
 // Dense split.  In a split launch the first d.nlane workgroups run the
 // per-lane pass (lane_particle: ~0 for queued particles, which they skip)
 // and the rest take the queued particles, a quad each, in queue order (a
@@ -830,16 +922,25 @@ __device__ __forceinline__ uint32_t lane_particle(const Dev& d) {
 }
 template <class F>
 __device__ __forceinline__ void for_dense_quads(const Dev& d, F f) {
-  const uint32_t qn = d.flags[kDenseSlot];
+  const uint32_t qn = min(d.flags[kDenseSlot], d.n);
   const uint32_t stride = (gridDim.x - d.nlane) * (kBlock / 4);
   for (uint32_t q = (blockIdx.x - d.nlane) * (kBlock / 4) + (threadIdx.x >> 2); q < qn;
        q += stride)
     f(d.dense[q]);  // a quad's lanes share q
 }
 
+// Whether sorted particle i's cell holds more than dense_cell particles
+// (the test the sort's queue used).
+__device__ __forceinline__ bool crowded(const Dev& d, uint32_t i) {
+  const uint32_t c = d.cellSorted[i];
+  return min(d.cellStart[c + 1], d.n) - d.cellStart[c] > d.dense_cell;
+}
+
 // Queues particle i (its list longer than the threshold) for the dense split.
 __device__ __forceinline__ void queue_dense(const Dev& d, uint32_t i, uint32_t cnt) {
-  if (cnt > d.dense_list) d.dense[atomicAdd(&d.flags[kDenseSlot], 1u)] = i;
+  if (cnt <= d.dense_list) return;
+  const uint32_t slot = atomicAdd(&d.flags[kDenseSlot], 1u);
+  if (slot < d.n) d.dense[slot] = i;  // each particle once per step: slot < n
 }
 
 // ------------------------------------------------------- flat schedule --
@@ -880,14 +981,19 @@ __global__ __launch_bounds__(kBlock) void k_predict_into(Dev d, SimConst c, Scen
   if (valid) d.arrival[i] = slot;
 }
 
+// The dense split's queue counters start at zero in every step: the build
+// queue's before the rank pass fills it (here), the list queue's before the
+// build fills it (in k_rank).
 __global__ __launch_bounds__(kBlock) void k_scatter(Dev d) {
   const uint32_t i = flat_index();
   if (i < d.n) scatter_one(i, d);
+  if (i == 0 && d.dense_b) d.flags[kDenseBuildSlot] = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rank(Dev d, const float4* pred) {
   const uint32_t i = flat_index();
   if (i < d.n) rank_one(i, d, pred);
+  if (i == 0 && d.dense) d.flags[kDenseSlot] = 0u;
 }
 
 template <bool MULTI>
@@ -905,7 +1011,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_build(Dev d, float h2, SceneGrid
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneGrid g0,
                                                        float4* out) {
-  const uint32_t i = gather_index();
+  if (!MULTI && dense_quads_part(d)) {
+    // Build split: the particles of crowded cells, a quad each.
+    const uint32_t qn = min(d.flags[kDenseBuildSlot], d.n);
+    const uint32_t stride = (gridDim.x - d.nlane) * (kBlock / 4);
+    for (uint32_t q = (blockIdx.x - d.nlane) * (kBlock / 4) + (threadIdx.x >> 2); q < qn;
+         q += stride) {
+      const uint32_t i = d.dense_b[q];  // a quad's lanes share q
+      nbr_quad_one(i, d, c.h2, g0);
+      __threadfence_block();  // the quad's list stores before its loads
+      lambda_quad_one<false>(i, d, c, d.predQ, out);
+      if ((threadIdx.x & 3u) == 0 && d.dense) queue_dense(d, i, d.ncnt[i]);
+    }
+    return;
+  }
+  uint32_t i = gather_index();
+  if (!MULTI && d.nlane) {
+    i = xcd_block(blockIdx.x, d.nlane) * kBlock + threadIdx.x;
+    if (i < d.n && crowded(d, i)) return;  // built by a quad
+  }
   if (i >= d.n) return;
   nbr_one(i, d, c.h2, grid_of<MULTI>(i, g0, d));
   lambda_one<false>(i, d, c, d.predQ, out);
@@ -1355,86 +1479,9 @@ __global__ __launch_bounds__(kBlock) void k_deltap_quad(Dev d, SimConst c, Scene
   if (i < d.n) deltap_quad_one<LAST, NT>(i, d, c, g, in);
 }
 
-// The neighbour build with a quad per particle (single scene; nbr_one's
-// rows, pruning and order).  The quad walks each row 16 candidates at a
-// time: lane k tests candidates k, k + 4, k + 8, k + 12, and four DPP
-// broadcasts give every lane the quad's 16-bit hit mask in candidate order.
-// A hit lane stores its entry at slot cnt + (hits before it in the mask),
-// so the list is nbr_one's.  The slots past the end of the last chunk hold
-// i (a particle index, as the readers' gathers need; their terms are not
-// added).  The first lambda pass follows as its own quad launch.
-__device__ __forceinline__ uint32_t quad_or(uint32_t m) {
-  return (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 0)) |
-         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 1)) |
-         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 2)) |
-         (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 3));
-}
-
 __global__ __launch_bounds__(kBlock) void k_nbr_quad(Dev d, float h2, SceneGrid g) {
   const uint32_t i = quad_index();
-  if (i >= d.n) return;  // a quad's lanes leave together
-  const uint32_t k = threadIdx.x & 3u;
-  const float4 xi = d.predQ[i];
-  int cc[3];
-  cell_lin(g, make_float3(xi.x, xi.y, xi.z), cc);
-  const float cell = 1.0f / g.inv_cell;
-  const float hm2 = h2 * 1.0201f;  // (1.01 h)^2
-  const float4* __restrict__ pred = d.predQ;
-  const uint32_t cap = (uint32_t)d.K4 * kChunk;
-  auto hit = [&](const float4& xj) -> uint32_t {
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float d2 = rx * rx + ry * ry + rz * rz;
-    return d2 <= h2 ? 1u : 0u;
-  };
-  auto put = [&](uint32_t slot, uint32_t j) {
-    if (slot < cap) d.nbr[nbr_chunk(i, (int)(slot >> 2), d.K4) + (slot & 3u)] = j;
-  };
-  uint32_t cnt = 0;
-  const float gxl = face_gap(xi.x, g.lo[0], cell, cc[0], -1);
-  const float gxr = face_gap(xi.x, g.lo[0], cell, cc[0], 1);
-  for (int dz = -1; dz <= 1; dz++) {
-    const int z = cc[2] + dz;
-    if (z < 0 || z >= g.dims[2]) continue;
-    const float gz = face_gap(xi.z, g.lo[2], cell, cc[2], dz);
-    for (int dy = -1; dy <= 1; dy++) {
-      const int y = cc[1] + dy;
-      if (y < 0 || y >= g.dims[1]) continue;
-      const float gy = face_gap(xi.y, g.lo[1], cell, cc[1], dy);
-      const float yz2 = gy * gy + gz * gz;
-      if (yz2 > hm2) continue;
-      const int x0 = (cc[0] > 0 && gxl * gxl + yz2 <= hm2) ? cc[0] - 1 : cc[0];
-      const int x1 = (cc[0] + 1 < g.dims[0] && gxr * gxr + yz2 <= hm2) ? cc[0] + 1 : cc[0];
-      const uint32_t row =
-          g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
-      const uint32_t s = d.cellStart[row + (uint32_t)x0];
-      const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
-      for (uint32_t jb = s; jb < e; jb += 16) {
-        // Up to 15 candidates past the range are read (the x* arrays are
-        // padded by 64 entries); their bits are dropped.
-        uint32_t mine = 0;
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
-          const uint32_t j = jb + k + 4u * u;
-          mine |= (hit(at(pred, j)) & (uint32_t)(j < e && j != i)) << (4u * u + k);
-        }
-        const uint32_t m = quad_or(mine);
-#pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
-          const uint32_t t = 4u * u + k;
-          if ((m >> t) & 1u) put(cnt + (uint32_t)__builtin_popcount(m & ((1u << t) - 1u)), jb + t);
-        }
-        cnt += (uint32_t)__builtin_popcount(m);
-      }
-    }
-  }
-  // Slots cnt .. end of its chunk: lane k fills slot cnt + k if that is in
-  // the same chunk.
-  const uint32_t pad = cnt + k;
-  if ((cnt & 3u) && (pad >> 2) == (cnt >> 2)) put(pad, i);
-  if (k == 0) {
-    if (cnt > cap) atomicMax(&d.flags[0], cnt);
-    d.ncnt[i] = min(cnt, cap);
-  }
+  if (i < d.n) nbr_quad_one(i, d, h2, g);  // a quad's lanes leave together
 }
 
 // The vorticity confinement and commit (no next prediction) with a quad per
@@ -1464,6 +1511,12 @@ constexpr int kSortSmallBlock = 1024;
 __global__ __launch_bounds__(kSortSmallBlock) void k_sort_small(Dev d, const float4* pred,
                                                                 uint32_t ncell) {
   __shared__ uint32_t wsum[kSortSmallBlock / 64 + 1];
+  // The dense split's queue counters, as k_scatter and k_rank reset them
+  // (visible to the rank phase's atomics after the barriers below).
+  if (threadIdx.x == 0) {
+    if (d.dense_b) d.flags[kDenseBuildSlot] = 0u;
+    if (d.dense) d.flags[kDenseSlot] = 0u;
+  }
   block_scan<kSortSmallBlock>(d.cellCount, d.cellStart, ncell, 0u, wsum);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < d.n; i += kSortSmallBlock) scatter_one(i, d);
@@ -1635,10 +1688,6 @@ static bool quad(const Ctx& x, ListPass p) { return lanes(x, p) == 4; }
 static unsigned wide_grid(const Ctx& x) { return (unsigned)x.n; }
 static unsigned quad_grid(const Ctx& x) { return blocks_for(x.n * 4); }
 
-// The dense queue starts empty at every build.
-static hipError_t clear_dense(const Ctx& x, const Dev& d) {
-  return d.dense ? launch_fill_u32(x.d.flags + kDenseSlot, 1, 0u, x.stream) : hipSuccess;
-}
 // A per-lane pass over the particles, split with the dense queue when x
 // has one: the per-lane workgroups, then as many quad workgroups.
 static Dev split_dev(const Ctx& x, ListPass p, unsigned* grid) {
@@ -1653,8 +1702,6 @@ static Dev split_dev(const Ctx& x, ListPass p, unsigned* grid) {
 
 hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
   const Dev d = dev_of(x);
-  const hipError_t e = clear_dense(x, d);
-  if (e != hipSuccess) return e;
   if (quad(x, kPassBuild)) {
     k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
     return hipGetLastError();
@@ -1668,8 +1715,6 @@ hipError_t launch_nbr_build(Ctx& x, const SimConst& c) {
 
 hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
   const Dev d = dev_of(x);
-  const hipError_t e = clear_dense(x, d);
-  if (e != hipSuccess) return e;
   if (quad(x, kPassBuild)) {
     // The quad build, then the first lambda pass over the finished lists.
     k_nbr_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(d, c.h2, x.grids[0]);
@@ -1683,10 +1728,15 @@ hipError_t launch_nbr_lambda(Ctx& x, const SimConst& c, float4* out) {
       k_nbr_lambda_wide<false><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], out);
     return hipGetLastError();
   }
-  if (multi(x))
+  if (multi(x)) {
     k_nbr_lambda<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
-  else
+  } else if (d.dense_b) {
+    Dev ds = d;  // build split: the per-lane workgroups, then as many quad workgroups
+    ds.nlane = blocks_for(x.n);
+    k_nbr_lambda<false><<<2u * ds.nlane, kBlock, 0, x.stream>>>(ds, c, x.grids[0], out);
+  } else {
     k_nbr_lambda<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], out);
+  }
   return hipGetLastError();
 }
 

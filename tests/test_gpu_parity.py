@@ -268,14 +268,18 @@ def test_wide_quad_and_lane_passes_equal(orc, spacing_over_h):
         assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
 
 
-@pytest.mark.parametrize("dense_list", ["60", "120", "0"])
-def test_dense_split_bit_exact(orc, dense_list):
+@pytest.mark.parametrize("dense_list,dense_cell", [("60", "24"), ("120", "48"), ("100000", "24"),
+                                                    ("0", "0")])
+def test_dense_split_bit_exact(orc, dense_list, dense_cell):
     """One lane per particle with the dense split: the build queues the
     particles with more than PBF_DENSE_LIST neighbours (a block at spacing
     0.3 h holds about 150 at first and 98 at most after three steps, so 60
-    queues many particles at every step, 120 only in the first steps; 0
-    turns the split off) and the lambda, delta-p and XSPH launches give them
-    to quad workgroups.  Three steps equal the oracle's."""
+    queues many particles at every step, 120 only in the first steps,
+    100000 none) and the lambda, delta-p and XSPH launches give them to quad
+    workgroups; the sort queues the particles of cells holding more than
+    PBF_DENSE_CELL (about 37 per cell at this spacing: 24 queues most of
+    them, 48 a few) and the build + first lambda launch builds their lists
+    on quads; 0, 0 turns both off.  Three steps equal the oracle's."""
     sc = scenes.dam_block()
     h = sc.params.h
     s = 0.3 * h
@@ -284,7 +288,8 @@ def test_dense_split_bit_exact(orc, dense_list):
     v = np.random.default_rng(5).uniform(-0.2, 0.2, x.shape).astype(np.float32)
     p = sc.params.replace(max_neighbors=512)
     xg, vg, rg = orc.step(p, x, v, 3, orc.NBR_GRID)
-    env = {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": dense_list}
+    env = {"PBF_WIDE_MAX": "0", "PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": dense_list,
+           "PBF_DENSE_CELL": dense_cell}
     ((gx, gv, gr), (nbar, nmax)), = run_modes(p, x, v, 3, (env,))
     assert nmax > 60
     assert bits_equal(gx, xg) and bits_equal(gv, vg) and bits_equal(gr, rg)
