@@ -776,8 +776,9 @@ __device__ __forceinline__ void density_one(uint32_t i, const Dev& d, const SimC
 }
 
 // ----------------------------------------- one particle on one quad --
-// The lambda, delta-p and XSPH passes of particle i on the four lanes of a
-// quad (quad_sums); every lane of the quad calls them.
+// The list build and the lambda, delta-p and XSPH passes of particle i on
+// the four lanes of a quad; every lane of the quad calls them.  The quad
+// kernels and the dense split's quad workgroups share them.
 __device__ __forceinline__ uint32_t quad_index() {
   return (xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) >> 2;
 }
