@@ -76,9 +76,10 @@ struct DevState {
 //                         65,536:  -   / 12.6 / 15.2 131,072:  -   / 21.2 / 19.1
 //   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
 //   vorticity commit      32,768:  -   / 8.6 / 11.0   65,536:  -   / 10.2 / 11.1
-//                         98,304:  -   / 13.6 / 12.6  PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX
-// (quad_max) and PBF_RULE_<PASS>=wave_max,quad_max override them at context
-// creation; 0 turns a variant off.
+//                         98,304:  -   / 13.6 / 12.6
+// PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX (every quad_max) and
+// PBF_RULE_<PASS>=wave_max,quad_max override them at context creation; 0
+// turns a variant off.
 enum ListPass : int { kPassBuild, kPassLambda, kPassDeltap, kPassXsph, kPassVort, kListPasses };
 struct PassRule {
   int64_t wave_max;
@@ -93,19 +94,19 @@ constexpr PassRule kPassRules[kListPasses] = {
 };
 
 // Dense split: in single-scene states of at most kDenseMaxN particles whose
-// list build runs one lane per particle, the build queues the particles with more than dense_list
-// neighbours (count in flags[kDenseSlot]), and the per-lane lambda, delta-p
-// and XSPH launches leave them to quad workgroups appended to the same
-// launch.  A quad walks a list as four chunk streams at once (quad_sums), so
-// a compressed region's longest lists take a quarter of the dependent
-// rounds of the lone waves that otherwise set these launches' length; the
-// other particles keep one lane each.  Same bits (the quad sums are the
-// per-lane sums).  PBF_DENSE_LIST overrides the threshold (0: off).
-// Measured on the 262,144-particle dam break (200 steps): 1.32 ms per step
-// without, 0.97 at 128 (0.99 at 64, 1.07 at 256); uniform blocks of
-// 131,072 and 262,144 particles, with no list above 128, 1-1.6% slower for
-// the second half of workgroups, and the 4,096,000-particle run 0.6%
-// slower, hence the bound on n.
+// list build runs one lane per particle, the build queues the particles
+// with more than dense_list neighbours (count in flags[kDenseSlot]), and the
+// per-lane lambda, delta-p and XSPH launches leave them to quad workgroups
+// appended to the same launch.  A quad walks a list as four chunk streams
+// at once (quad_sums), so a compressed region's longest lists take a
+// quarter of the dependent rounds of the lone waves that otherwise set
+// these launches' length; the other particles keep one lane each.  Same
+// bits (the quad sums are the per-lane sums).  PBF_DENSE_LIST overrides the
+// threshold (0: off).  Measured on the 262,144-particle dam break (200
+// steps): 1.32 ms per step without, 0.97 at 128 (0.99 at 64, 1.07 at 256),
+// 0.93 with the build split below as well; uniform blocks of 131,072 and
+// 262,144 particles (no list above 128, no cell above 32) within 0.6%; the
+// 4,096,000-particle run 0.6% slower, hence the bound on n.
 constexpr uint32_t kDenseList = 128;
 constexpr int64_t kDenseMaxN = int64_t{1} << 20;
 constexpr int kDenseSlot = 12;
