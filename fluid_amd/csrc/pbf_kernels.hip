@@ -829,13 +829,14 @@ __device__ __forceinline__ void xsph_quad_one(uint32_t i, const Dev& d, const Si
 }
 
 // The neighbour build with a quad per particle (single scene; nbr_one's
-// rows, pruning and order).  The quad walks each row 16 candidates at a
-// time: lane k tests candidates k, k + 4, k + 8, k + 12, and four DPP
-// broadcasts give every lane the quad's 16-bit hit mask in candidate order.
+// rows, pruning and order).  The quad walks each row R candidates at a
+// time: lane k tests candidates k, k + 4, k + 8, ..., and four DPP
+// broadcasts give every lane the quad's R-bit hit mask in candidate order.
 // A hit lane stores its entry at slot cnt + (hits before it in the mask),
 // so the list is nbr_one's.  The slots past the end of the last chunk hold
 // i (a particle index, as the readers' gathers need; their terms are not
-// added).  The first lambda pass follows as its own quad launch.
+// added).  The first lambda pass follows on the same quad (the build
+// split) or as its own quad launch.
 __device__ __forceinline__ uint32_t quad_or(uint32_t m) {
   return (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 0)) |
          (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 1)) |
@@ -843,8 +844,14 @@ __device__ __forceinline__ uint32_t quad_or(uint32_t m) {
          (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 3));
 }
 
+// R candidates per round (16 or 32: four or eight per lane); the build
+// split's crowded particles, whose rows are long, take 32 (the dam break
+// 0.927 -> 0.920 ms per step), the quad build of mid-size states 16 (32 cost
+// up to 3% there).
+template <int R>
 __device__ __forceinline__ void nbr_quad_one(uint32_t i, const Dev& d, float h2,
                                              const SceneGrid& g) {
+  static_assert(R == 16 || R == 32, "16 or 32 candidates per round");
   const uint32_t k = threadIdx.x & 3u;
   const float4 xi = d.predQ[i];
   int cc[3];
@@ -880,18 +887,18 @@ __device__ __forceinline__ void nbr_quad_one(uint32_t i, const Dev& d, float h2,
           g.cell_base + ((uint32_t)z * (uint32_t)g.dims[1] + (uint32_t)y) * (uint32_t)g.dims[0];
       const uint32_t s = d.cellStart[row + (uint32_t)x0];
       const uint32_t e = min(d.cellStart[row + (uint32_t)x1 + 1u], d.n);
-      for (uint32_t jb = s; jb < e; jb += 16) {
-        // Up to 15 candidates past the range are read (the x* arrays are
+      for (uint32_t jb = s; jb < e; jb += R) {
+        // Up to R - 1 candidates past the range are read (the x* arrays are
         // padded by 64 entries); their bits are dropped.
         uint32_t mine = 0;
 #pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
+        for (uint32_t u = 0; u < R / 4; u++) {
           const uint32_t j = jb + k + 4u * u;
           mine |= (hit(at(pred, j)) & (uint32_t)(j < e && j != i)) << (4u * u + k);
         }
         const uint32_t m = quad_or(mine);
 #pragma unroll
-        for (uint32_t u = 0; u < 4; u++) {
+        for (uint32_t u = 0; u < R / 4; u++) {
           const uint32_t t = 4u * u + k;
           if ((m >> t) & 1u) put(cnt + (uint32_t)__builtin_popcount(m & ((1u << t) - 1u)), jb + t);
         }
@@ -1019,7 +1026,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneG
     for (uint32_t q = (blockIdx.x - d.nlane) * (kBlock / 4) + (threadIdx.x >> 2); q < qn;
          q += stride) {
       const uint32_t i = d.dense_b[q];  // a quad's lanes share q
-      nbr_quad_one(i, d, c.h2, g0);
+      nbr_quad_one<32>(i, d, c.h2, g0);
       __threadfence_block();  // the quad's list stores before its loads
       lambda_quad_one<false>(i, d, c, d.predQ, out);
       if ((threadIdx.x & 3u) == 0 && d.dense) queue_dense(d, i, d.ncnt[i]);
@@ -1482,7 +1489,7 @@ __global__ __launch_bounds__(kBlock) void k_deltap_quad(Dev d, SimConst c, Scene
 
 __global__ __launch_bounds__(kBlock) void k_nbr_quad(Dev d, float h2, SceneGrid g) {
   const uint32_t i = quad_index();
-  if (i < d.n) nbr_quad_one(i, d, h2, g);  // a quad's lanes leave together
+  if (i < d.n) nbr_quad_one<16>(i, d, h2, g);  // a quad's lanes leave together
 }
 
 // The vorticity confinement and commit (no next prediction) with a quad per
