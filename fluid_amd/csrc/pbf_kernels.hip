@@ -844,10 +844,11 @@ __device__ __forceinline__ uint32_t quad_or(uint32_t m) {
          (uint32_t)__float_as_int(quad_lane(__int_as_float((int)m), 3));
 }
 
-// R candidates per round (16 or 32: four or eight per lane); the build
-// split's crowded particles, whose rows are long, take 32 (the dam break
-// 0.927 -> 0.920 ms per step), the quad build of mid-size states 16 (32 cost
-// up to 3% there).
+// R candidates per round (16 or 32: four or eight per lane).  Both uses
+// take 16: 32 cost mid-size quad builds up to 3%, and in the build split
+// (the dam break 0.927 -> 0.920 ms per step) it raised the per-lane build
+// kernel that carries it from 72 to 107 VGPRs, which slowed the 4M run's
+// build (4.08 -> 4.13 ms per step).
 template <int R>
 __device__ __forceinline__ void nbr_quad_one(uint32_t i, const Dev& d, float h2,
                                              const SceneGrid& g) {
@@ -1026,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_lambda(Dev d, SimConst c, SceneG
     for (uint32_t q = (blockIdx.x - d.nlane) * (kBlock / 4) + (threadIdx.x >> 2); q < qn;
          q += stride) {
       const uint32_t i = d.dense_b[q];  // a quad's lanes share q
-      nbr_quad_one<32>(i, d, c.h2, g0);
+      nbr_quad_one<16>(i, d, c.h2, g0);
       __threadfence_block();  // the quad's list stores before its loads
       lambda_quad_one<false>(i, d, c, d.predQ, out);
       if ((threadIdx.x & 3u) == 0 && d.dense) queue_dense(d, i, d.ncnt[i]);
