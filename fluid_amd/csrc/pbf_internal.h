@@ -58,8 +58,8 @@ struct DevState {
 
 // Lanes per particle of each list pass, by state size: one wave (64 lanes)
 // for states of at most wave_max particles, one DPP quad (4 lanes; single
-// scenes; for the vorticity pass only its launch that does not carry the
-// next step's prediction) up to quad_max, else one lane.
+// scenes; for the vorticity pass that carries the next step's prediction
+// only up to kVortPredictQuadMax) up to quad_max, else one lane.
 // Same bits in every case.  The bounds sit between the sizes where the
 // kernel time per launch crosses over (rocprofv3 kernel traces of lattice
 // blocks at 0.5 h spacing, 4 iterations; microseconds per launch, wave /
@@ -75,8 +75,8 @@ struct DevState {
 //   XSPH                   8,192:  5.5 / 7.8 / 13.9   16,384:  8.3 / 7.7 / 13.9
 //                         65,536:  -   / 12.6 / 15.2 131,072:  -   / 21.2 / 19.1
 //   vorticity              8,192: 10.3 / -   / 11.6   16,384: 13.8 / -    / 11.9
-//   vorticity commit      32,768:  -   / 8.6 / 11.0   65,536:  -   / 10.2 / 11.1
-//                         98,304:  -   / 13.6 / 12.6
+//   vorticity commit      16,384:  -   / 5.7 / 10.9   32,768:  -   / 6.6 / 11.3
+//                         65,536:  -   / 8.9 / 11.9   98,304:  -   / 13.6 / 12.6
 // PBF_WIDE_MAX (every pass's wave_max), PBF_QUAD_MAX (every quad_max) and
 // PBF_RULE_<PASS>=wave_max,quad_max override them at context creation; 0
 // turns a variant off.
@@ -92,6 +92,12 @@ constexpr PassRule kPassRules[kListPasses] = {
     {12288, 114688},  // XSPH
     {12288, 81920},   // vorticity (+ commit, + prediction)
 };
+// The vorticity pass that carries the next step's prediction takes its quad
+// only up to this many particles: its histogram slots cost a quad one
+// atomic per particle, where a lane per particle takes one per run of equal
+// cells (microseconds per launch, quad / lane: 16,384 9.4 / 12.1, 32,768
+// 11.8 / 12.2, 65,536 17.5 / 12.5).
+constexpr int64_t kVortPredictQuadMax = 24576;
 
 // Dense split: in single-scene states of at most kDenseMaxN particles whose
 // list build runs one lane per particle, the build queues the particles

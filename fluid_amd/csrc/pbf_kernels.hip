@@ -1495,7 +1495,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_quad(Dev d, float h2, SceneGrid 
 
 // The vorticity confinement and commit (no next prediction) with a quad per
 // particle.
-__global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, const float4* in) {
+// With PREDICT, the next step's prediction from the committed state too,
+// and its histogram slot (one atomic per particle: the sort ranks a cell's
+// particles by id, whatever their slots).
+template <bool PREDICT>
+__global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, SceneGrid g,
+                                                      const float4* in) {
   const uint32_t i = quad_index();
   if (i >= d.n) return;
   const float4 xi = in[i];
@@ -1503,7 +1508,15 @@ __global__ __launch_bounds__(kBlock) void k_vort_quad(Dev d, SimConst c, const f
   quad_sums<3>(
       i, (int)d.ncnt[i], d.K4, d.nbr, [&](uint32_t j) { return at(in, j); },
       [&](uint32_t, const float4& xj, float* t) { vort_terms(c, xi, xj, t); }, s);
-  if ((threadIdx.x & 3u) == 0) vort_finish(i, d, c, xi, s[0], s[1], s[2], nullptr, nullptr, true);
+  if ((threadIdx.x & 3u) != 0) return;
+  if (PREDICT) {
+    float4 x, v;
+    vort_finish(i, d, c, xi, s[0], s[1], s[2], &x, &v, false);
+    const uint32_t cl = predict_from<true>(i, d, c, g, x, v, d.vort);
+    d.arrival[i] = atomicAdd(&d.cellCount[cl], 1u);
+  } else {
+    vort_finish(i, d, c, xi, s[0], s[1], s[2], nullptr, nullptr, true);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_xsph_quad(Dev d, SimConst c, float4* out) {
@@ -1871,6 +1884,10 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
     const hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
     if (e != hipSuccess) return e;
   }
+  if (quad(x, kPassVort) && x.n <= kVortPredictQuadMax) {
+    k_vort_quad<true><<<quad_grid(x), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
+    return hipGetLastError();
+  }
   if (wide(x, kPassVort)) {
     if (multi(x))
       k_vort_wide<true, true><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
@@ -1887,7 +1904,7 @@ hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* i
 
 hipError_t launch_vort_commit(Ctx& x, const SimConst& c, const float4* in) {
   if (quad(x, kPassVort)) {
-    k_vort_quad<<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, in);
+    k_vort_quad<false><<<quad_grid(x), kBlock, 0, x.stream>>>(dev_of(x), c, x.grids[0], in);
     return hipGetLastError();
   }
   if (wide(x, kPassVort)) {
