@@ -148,8 +148,9 @@ def test_mid_size_rollout(orc):
     steps with checkpoints every 2: forward bit-exact with the oracle, the
     replay equal to the forward pass, FP64 central differences along dL/dv0
     and one random direction within 1e-3, and a rollout created with
-    PBF_QUAD_MAX=0 (one lane per particle throughout) gives the same bits
-    for the forward pass and the gradients."""
+    PBF_QUAD_MAX=0 (one lane per particle throughout), and one with the
+    dense splits at low thresholds (most particles on the split's quads),
+    give the same bits for the forward pass and the gradients."""
     sc, target = scenes.diff_block(n_side=36, steps=5)
     assert sc.n == 46_656
     k = float(np.float32(sc.params.k_tensile))
@@ -161,18 +162,21 @@ def test_mid_size_rollout(orc):
     _, contacts = r.stats()
     r.close()
     assert contacts == 0
-    os.environ["PBF_QUAD_MAX"] = "0"
-    try:
-        r1 = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=2)
-    finally:
-        os.environ.pop("PBF_QUAD_MAX", None)
-    xT1, vT1 = r1.forward(sc.x, sc.v, k)
-    xb1, vb1, kb1 = r1.backward(xbar)
-    assert r1.replay_mismatches() == 0
-    r1.close()
-    for a, b in ((xT, xT1), (vT, vT1), (xb0, xb1), (vb0, vb1)):
-        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
-    assert kb == kb1
+    for env in ({"PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "0"},
+                {"PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "8", "PBF_DENSE_CELL": "2"}):
+        os.environ.update(env)
+        try:
+            r1 = Rollout(sc.params, sc.n, sc.steps, checkpoint_every=2)
+        finally:
+            for key in env:
+                os.environ.pop(key, None)
+        xT1, vT1 = r1.forward(sc.x, sc.v, k)
+        xb1, vb1, kb1 = r1.backward(xbar)
+        assert r1.replay_mismatches() == 0
+        r1.close()
+        for a, b in ((xT, xT1), (vT, vT1), (xb0, xb1), (vb0, vb1)):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert kb == kb1
     ox, ov, _ = orc.step(sc.params, sc.x, sc.v, sc.steps, orc.NBR_GRID)
     assert np.array_equal(xT.view(np.uint32), ox.view(np.uint32))
     assert np.array_equal(vT.view(np.uint32), ov.view(np.uint32))

@@ -79,8 +79,11 @@ def test_falling_slab_on_sphere_quad_and_lane_passes_bit_exact(orc_mesh):
     """A 48x20x48 slab (46,080 particles, above the wave-per-particle range)
     falling onto the sphere: the oracle's bits after 15 and 30 steps, with
     the delta-p pass and its collision calls run with a quad per particle
-    (the default rule at this size) and, in a second context created with
-    PBF_QUAD_MAX=0, with one lane per particle."""
+    (the default rule at this size), in a second context created with
+    PBF_QUAD_MAX=0 and PBF_DENSE_LIST=0 with one lane per particle, and in a
+    third with one lane and the dense splits at thresholds low enough that
+    most particles take the split's quads (PBF_DENSE_LIST=8,
+    PBF_DENSE_CELL=2)."""
     import os
 
     orc = orc_mesh
@@ -94,21 +97,26 @@ def test_falling_slab_on_sphere_quad_and_lane_passes_bit_exact(orc_mesh):
     m = m + (mesh.default_skin(m[0], (0, 0, 0), (1, 1, 1)),)
     orc.set_mesh(*m)
     v = np.zeros_like(x)
-    os.environ["PBF_QUAD_MAX"] = "0"
-    try:
-        lane = Particles(p)
-    finally:
-        os.environ.pop("PBF_QUAD_MAX", None)
-    with Particles(p) as c, lane:
-        for ctx in (c, lane):
+    def make(env):
+        os.environ.update(env)
+        try:
+            return Particles(p)
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+
+    lane = make({"PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "0"})
+    split = make({"PBF_QUAD_MAX": "0", "PBF_DENSE_LIST": "8", "PBF_DENSE_CELL": "2"})
+    with Particles(p) as c, lane, split:
+        for ctx in (c, lane, split):
             ctx.set_mesh(*m)
             ctx.load(x, v)
         ox, ov = x, v
         for _ in range(2):
-            c.time_step(nsteps=15)
-            lane.time_step(nsteps=15)
+            for ctx in (c, lane, split):
+                ctx.time_step(nsteps=15)
             ox, ov, _ = orc.step(p, ox, ov, 15, orc.NBR_GRID)
-            for ctx in (c, lane):
+            for ctx in (c, lane, split):
                 gx, gv, _ = ctx.read()
                 assert bits_equal(gx, ox) and bits_equal(gv, ov)
     r = np.linalg.norm(gx - np.array([0.5, 0.3, 0.5]), axis=1)
