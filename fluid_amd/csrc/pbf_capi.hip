@@ -304,6 +304,13 @@ int mark(Ctx* x, PassEvents* pe, int pass, bool begin) {
 int run_step(Ctx* x, const SimConst& c, int stage, PassEvents* pe = nullptr,
              bool predicted = false, bool predict_next = false) {
   pbf::DevState& d = x->d;
+  // A full step neither stores the advanced velocity nor sorts it; the debug
+  // stops keep both (they read them back).  Restored on every return.
+  struct SortedV {
+    Ctx* x;
+    ~SortedV() { x->sorted_v = true; }
+  } restore{x};
+  x->sorted_v = stage != 0;
   if (!predicted) PASS(PBF_PASS_PREDICT, pbf::launch_predict(*x, c));
   // A prediction fused into the previous step's last pass sits in vort.
   if (x->small_sort && pbf::sort_in_one_workgroup(*x)) {

@@ -162,6 +162,11 @@ struct Ctx {
                                  kPassRules[4]};
   uint32_t dense_list = kDenseList;  // dense split threshold (0: off)
   uint32_t dense_cell = kDenseCell;  // build split threshold (0: off)
+  // Whether the prediction stores the advanced velocity in velA and the sort
+  // moves it into velB with the particle.  A full step reads neither (its
+  // velocity comes from x* - x and the commit rewrites velA), so run_step
+  // clears this for full steps; the debug stops and the adjoint keep it.
+  bool sorted_v = true;
   // Density-field output on the device, kept between calls, and two pinned
   // host slabs through which large results return (allocated on first use).
   float* field_dev = nullptr;

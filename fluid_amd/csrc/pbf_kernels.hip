@@ -60,6 +60,9 @@ struct Dev {
   // threshold on the cell's population.
   uint32_t* dense_b;
   uint32_t dense_cell;
+  // Nonzero: the prediction stores the advanced v in velA and the sort moves
+  // it to velB (read only by debug stops and the adjoint).
+  uint32_t move_v;
   uint32_t n;
   int K4;
 };
@@ -94,6 +97,7 @@ Dev dev_of(const Ctx& x) {
   d.dense_list = x.dense_list;
   d.dense_b = dense_split(x) && x.dense_cell > 0 ? s.dense_b : nullptr;
   d.dense_cell = x.dense_cell;
+  d.move_v = x.sorted_v ? 1u : 0u;
   d.nlane = 0;
   d.n = (uint32_t)x.n;
   d.K4 = k4_of(x);
@@ -275,7 +279,7 @@ __device__ __forceinline__ uint32_t predict_from(uint32_t i, const Dev& d, const
   if (MOVE) {
     v.y = v.y - c.gdt;
     p = clamp_response(g, c, d.mesh, p, make_float3(v.x, v.y, v.z));
-    d.velA[i] = v;
+    if (d.move_v) d.velA[i] = v;
   }
   pred[i] = make_float4(p.x, p.y, p.z, 0.0f);
   const uint32_t cl = cell_lin(g, p);
@@ -327,13 +331,14 @@ __device__ __forceinline__ void scatter_one(uint32_t i, const Dev& d) {
 // Rank of a particle inside its cell = number of cell members with a smaller
 // id; the gather target is cellStart + rank, which makes the order (cell, id)
 // independent of atomic arrival order.
+// Moves x, x* and (with d.move_v) v to the sorted slot.
 // Everything the particle moves is loaded before the count, and the cell's
 // ids are read four at a time with independent loads, so the loads of one
 // particle overlap instead of waiting on each other.
 __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d,
                                          const float4* __restrict__ pred) {
   const float4 x = d.posA[i];
-  const float4 v = d.velA[i];
+  const float4 v = d.move_v ? d.velA[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 q = pred[i];
   const uint32_t c = d.cell[i];
   const uint32_t s = d.cellStart[c];
@@ -354,7 +359,7 @@ __device__ __forceinline__ void rank_one(uint32_t i, const Dev& d,
   const uint32_t dst = s + rank;
   if (dst >= d.n) return;
   d.posB[dst] = x;
-  d.velB[dst] = v;
+  if (d.move_v) d.velB[dst] = v;
   d.predQ[dst] = q;
   d.cellSorted[dst] = c;
   // Build split: a particle of a crowded cell gets its list from a quad.
