@@ -251,7 +251,9 @@ hipError_t launch_verify_math(float b, float rb, int div_e_lo, int div_e_hi, uin
 SimConst consts_for(const Ctx& x, float dt);
 int persistent_max_blocks_per_cu(int block);
 hipError_t launch_nbr_stats(Ctx& x, unsigned long long* sum, uint32_t* mx);
-hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
-                           hipStream_t s);
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total; with clear_in
+// the counts are zero afterwards.
+hipError_t launch_scan_u32(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
+                           hipStream_t s, bool clear_in = false);
 
 }  // namespace pbf

@@ -124,19 +124,24 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
 
-// Block-local exclusive prefix of a 1024-element tile; tile totals to partials.
-__global__ __launch_bounds__(kBlock) void k_scan_block(const uint32_t* __restrict__ in,
+// Block-local exclusive prefix of a 1024-element tile; tile totals to
+// partials.  With clear, the counts are set to zero once read.
+__global__ __launch_bounds__(kBlock) void k_scan_block(uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, uint32_t n,
-                                                       uint32_t* __restrict__ partials) {
+                                                       uint32_t* __restrict__ partials, bool clear) {
   __shared__ uint32_t wsum[kBlock / 64];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   if (base + kScanItems <= n) {
     uint4 q = *reinterpret_cast<const uint4*>(in + base);
     v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    if (clear) *reinterpret_cast<uint4*>(in + base) = make_uint4(0u, 0u, 0u, 0u);
   } else {
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? in[base + k] : 0u;
+    if (clear)
+      for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) in[base + k] = 0u;
   }
   uint32_t t = (v[0] + v[1]) + (v[2] + v[3]);
   uint32_t incl = wave_incl_scan(t);
@@ -1658,10 +1663,10 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
-                           hipStream_t s) {
+hipError_t launch_scan_u32(uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials,
+                           hipStream_t s, bool clear_in) {
   const uint32_t nb = (n + kScanTile - 1) / kScanTile;
-  if (nb > 0) k_scan_block<<<nb, kBlock, 0, s>>>(in, out, n, partials);
+  if (nb > 0) k_scan_block<<<nb, kBlock, 0, s>>>(in, out, n, partials, clear_in);
   k_scan_partials<<<1, 1024, 0, s>>>(partials, nb, out, n);
   if (nb > 1) k_scan_add<<<blocks_for(n), kBlock, 0, s>>>(out, n, partials);
   return hipGetLastError();
@@ -1681,7 +1686,9 @@ hipError_t launch_predict(Ctx& x, const SimConst& c) {
 }
 
 hipError_t launch_scan(Ctx& x) {
-  return launch_scan_u32(x.d.cellCount, x.d.cellStart, x.ncell, x.d.blockSums, x.stream);
+  // The histogram is spent once scanned: the tile pass clears it for the
+  // next prediction, which then needs no fill launch of its own.
+  return launch_scan_u32(x.d.cellCount, x.d.cellStart, x.ncell, x.d.blockSums, x.stream, true);
 }
 
 bool sort_in_one_workgroup(const Ctx& x) {
@@ -1867,27 +1874,23 @@ hipError_t launch_unpack_record(Ctx& x, float4* pos, float4* vel) {
 }
 
 hipError_t launch_vort_commit_predict(Ctx& x, const SimConst& c, const float4* in) {
-  // After the single-workgroup sort the histogram is already zero.
-  const bool zeroed = x.small_sort && sort_in_one_workgroup(x);
+  // Both sorts leave the histogram zero (the single-workgroup sort and the
+  // general scan clear it once read), so the next prediction needs no fill.
+  const bool small = x.small_sort && sort_in_one_workgroup(x);
   const Dev d = dev_of(x);
-  if (wide(x, kPassVort) && !zeroed) {
+  if (wide(x, kPassVort) && !small) {
     // Beyond the smallest states the wide pass commits and the prediction
     // follows as its own launch (the serial tail of one lane per particle,
     // with one histogram atomic each, costs more than the launch).  It is
     // written where the fused prediction would be.
     k_vort_wide<false, false><<<wide_grid(x), kWide, 0, x.stream>>>(d, c, x.grids[0], in);
-    hipError_t e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if ((e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream)) != hipSuccess) return e;
     if (multi(x))
       k_predict_into<true><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], d.vort);
     else
       k_predict_into<false><<<blocks_for(x.n), kBlock, 0, x.stream>>>(d, c, x.grids[0], d.vort);
     return hipGetLastError();
-  }
-  if (!zeroed) {
-    const hipError_t e = launch_fill_u32(x.d.cellCount, x.ncell, 0u, x.stream);
-    if (e != hipSuccess) return e;
   }
   if (quad(x, kPassVort) && x.n <= kVortPredictQuadMax) {
     k_vort_quad<true><<<quad_grid(x), kBlock, 0, x.stream>>>(d, c, x.grids[0], in);
