@@ -632,6 +632,7 @@ PBF_API int pbf_destroy(pbf_ctx* c) {
   dfree(x->d.meshNode);
   dfree(x->d.meshFlag);
   stage_free(x);
+  if (x->flag_host) (void)hipHostFree(x->flag_host);
   dfree(x->field_dev);
   dfree(x->field_pts);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -737,9 +738,13 @@ namespace {
 
 // Waits for the stream, then reports (and clears) a neighbour-list overflow.
 int sync_check(Ctx* x) {
-  uint32_t over = 0;
-  int rc = copy_out(x, reinterpret_cast<const float*>(x->d.flags), &over, sizeof over);
-  if (rc != PBF_OK) return rc;
+  // The flag word returns through a pinned word of the context: a small
+  // copy into pageable memory is staged by the runtime on every call.
+  if (!x->flag_host) HIPCHK(x, hipHostMalloc((void**)&x->flag_host, 64, hipHostMallocDefault));
+  HIPCHK(x, hipMemcpyAsync(x->flag_host, x->d.flags, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  const uint32_t over = x->flag_host[0];
   if (over == 0) return PBF_OK;
   HIPCHK(x, pbf::launch_fill_u32(x->d.flags, 1, 0u, x->stream));
   HIPCHK(x, hipStreamSynchronize(x->stream));

@@ -175,6 +175,7 @@ struct Ctx {
   size_t field_pts_cap = 0;          // points
   float* stage_host[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  uint32_t* flag_host = nullptr;     // pinned word the per-call overflow check reads into
   hipGraph_t graph = nullptr;        // captured step (PBF_SCHED_GRAPH)
   hipGraphExec_t graph_exec = nullptr;
   float graph_dt = 0.0f;
